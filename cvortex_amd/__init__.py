@@ -1,0 +1,333 @@
+"""Python host interface of libcvortex.so (gfx950 direct-sum Biot-Savart).
+
+The functions mirror the reference cvortex C API (reference
+include/cvortex/libcvtx.h): same names without the ``cvtx_`` prefix, same
+argument meaning, records as numpy structured arrays (see
+:mod:`cvortex_amd.types`).  Every call goes through the C ABI of the in-tree
+``libcvortex.so``; there is no Python or CPU implementation of any pair
+interaction here, and loading fails loudly when the library has not been
+built.
+
+Errors: the public ``cvtx_*`` functions return void and signal failure by
+NaN results (plus a line on stderr); the wrappers here call the library's
+int-returning seam (``cvtx_hip_*``) instead and raise :class:`CvtxError`
+with the library's message when it returns -1.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .types import (F3D_DTYPE, P2D_DTYPE, P3D_DTYPE, REG_ID, REG_NAMES, VortFunc,
+                    as_records, pointer_array)
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcvortex.so")
+
+ACC_PLAIN = 0
+ACC_COMP = 1
+OUT_VEL, OUT_DVORT, OUT_VISC = 1, 2, 4
+
+_lib = None
+
+
+class CvtxError(RuntimeError):
+    pass
+
+
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+
+_SIGNATURES = {
+    # accelerator controls
+    "cvtx_initialise": (None, []),
+    "cvtx_finalise": (None, []),
+    "cvtx_information": (ctypes.c_char_p, []),
+    "cvtx_num_accelerators": (_I, []),
+    "cvtx_num_enabled_accelerators": (_I, []),
+    "cvtx_accelerator_name": (ctypes.c_char_p, [_I]),
+    "cvtx_accelerator_enabled": (_I, [_I]),
+    "cvtx_accelerator_enable": (None, [_I]),
+    "cvtx_accelerator_disable": (None, [_I]),
+    "cvtx_VortFunc_singular": (VortFunc, []),
+    "cvtx_VortFunc_winckelmans": (VortFunc, []),
+    "cvtx_VortFunc_planetary": (VortFunc, []),
+    "cvtx_VortFunc_gaussian": (VortFunc, []),
+    # public M2M (void)
+    "cvtx_P3D_M2M_vel": (None, [_VP, _I, _VP, _I, _VP, _VP, _F]),
+    "cvtx_P3D_M2M_dvort": (None, [_VP, _I, _VP, _I, _VP, _VP, _F]),
+    "cvtx_P3D_M2M_visc_dvort": (None, [_VP, _I, _VP, _I, _VP, _VP, _F, _F]),
+    "cvtx_F3D_M2M_vel": (None, [_VP, _I, _VP, _I, _VP]),
+    "cvtx_F3D_M2M_dvort": (None, [_VP, _I, _VP, _I, _VP]),
+    "cvtx_P2D_M2M_vel": (None, [_VP, _I, _VP, _I, _VP, _VP, _F]),
+    "cvtx_P2D_M2M_visc_dvort": (None, [_VP, _I, _VP, _I, _VP, _VP, _F, _F]),
+    # seam (int)
+    "cvtx_hip_P3D_M2M_vel": (_I, [_VP, _I, _VP, _I, _VP, _VP, _F]),
+    "cvtx_hip_P3D_M2M_dvort": (_I, [_VP, _I, _VP, _I, _VP, _VP, _F]),
+    "cvtx_hip_P3D_M2M_visc_dvort": (_I, [_VP, _I, _VP, _I, _VP, _VP, _F, _F]),
+    "cvtx_hip_F3D_M2M_vel": (_I, [_VP, _I, _VP, _I, _VP]),
+    "cvtx_hip_F3D_M2M_dvort": (_I, [_VP, _I, _VP, _I, _VP]),
+    "cvtx_hip_P2D_M2M_vel": (_I, [_VP, _I, _VP, _I, _VP, _VP, _F]),
+    "cvtx_hip_P2D_M2M_visc_dvort": (_I, [_VP, _I, _VP, _I, _VP, _VP, _F, _F]),
+    "cvtx_hip_P3D_M2M_fused": (_I, [_VP, _I, _VP, _I, _VP, _VP, _VP, _VP, _F, _F]),
+    "cvtx_hip_F3D_M2M_fused": (_I, [_VP, _I, _VP, _I, _VP, _VP]),
+    "cvtx_hip_P2D_M2M_fused": (_I, [_VP, _I, _VP, _I, _VP, _VP, _VP, _F, _F]),
+    # controls
+    "cvtx_hip_last_error": (ctypes.c_char_p, []),
+    "cvtx_hip_set_accumulation": (_I, [_I]),
+    "cvtx_hip_get_accumulation": (_I, []),
+    "cvtx_hip_stream": (_VP, []),
+    "cvtx_hip_synchronize": (_I, []),
+    # device-resident
+    "cvtx_hip_p3d_eval": (_I, [_VP, _VP]),
+    "cvtx_hip_p2d_eval": (_I, [_VP, _VP]),
+    "cvtx_hip_f3d_eval": (_I, [_VP, _VP]),
+    "cvtx_hip_p3d_eval_shape": (_I, [_VP, _I, _I, _VP]),
+    "cvtx_hip_p2d_eval_shape": (_I, [_VP, _I, _I, _VP]),
+    "cvtx_hip_f3d_eval_shape": (_I, [_VP, _I, _I, _VP]),
+    "cvtx_hip_autotune_p3d": (_I, [_VP, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                   ctypes.POINTER(_F)]),
+    "cvtx_hip_tune_save": (_I, [ctypes.c_char_p]),
+    "cvtx_hip_tune_load": (_I, [ctypes.c_char_p]),
+    "cvtx_hip_tune_clear": (None, []),
+}
+
+
+def lib():
+    """The loaded libcvortex.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CvtxError(f"{LIB_PATH} not found: build it first (python -c "
+                            "'import __graft_entry__ as g; g.build()' or make -C cvortex_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            if not hasattr(L, name):
+                continue
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    msg = lib().cvtx_hip_last_error()
+    return msg.decode() if msg else ""
+
+
+def _check(status, what):
+    if status != 0:
+        raise CvtxError(f"{what}: {last_error()}")
+
+
+# ----------------------------- controls -----------------------------
+
+def initialise():
+    lib().cvtx_initialise()
+
+
+def finalise():
+    lib().cvtx_finalise()
+
+
+def information():
+    return lib().cvtx_information().decode()
+
+
+def num_accelerators():
+    return lib().cvtx_num_accelerators()
+
+
+def num_enabled_accelerators():
+    return lib().cvtx_num_enabled_accelerators()
+
+
+def accelerator_name(i):
+    s = lib().cvtx_accelerator_name(i)
+    return s.decode() if s is not None else None
+
+
+def accelerator_enabled(i):
+    return lib().cvtx_accelerator_enabled(i)
+
+
+def accelerator_enable(i):
+    lib().cvtx_accelerator_enable(i)
+
+
+def accelerator_disable(i):
+    lib().cvtx_accelerator_disable(i)
+
+
+def set_accumulation(mode):
+    prev = lib().cvtx_hip_set_accumulation(mode)
+    if prev < 0:
+        raise CvtxError(f"invalid accumulation mode {mode}")
+    return prev
+
+
+def get_accumulation():
+    return lib().cvtx_hip_get_accumulation()
+
+
+def VortFunc_singular():
+    return lib().cvtx_VortFunc_singular()
+
+
+def VortFunc_winckelmans():
+    return lib().cvtx_VortFunc_winckelmans()
+
+
+def VortFunc_planetary():
+    return lib().cvtx_VortFunc_planetary()
+
+
+def VortFunc_gaussian():
+    return lib().cvtx_VortFunc_gaussian()
+
+
+def vortfunc(name):
+    """cvtx_VortFunc for a regularisation name."""
+    return {"singular": VortFunc_singular, "winckelmans": VortFunc_winckelmans,
+            "planetary": VortFunc_planetary, "gaussian": VortFunc_gaussian}[name]()
+
+
+def _kref(kernel):
+    if isinstance(kernel, str):
+        kernel = vortfunc(kernel)
+    return kernel, ctypes.cast(ctypes.pointer(kernel), ctypes.c_void_p)
+
+
+def _ptrs(records, dtype):
+    recs = as_records(records, dtype)
+    p, keep = pointer_array(recs)
+    return p, keep
+
+
+def _vp(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+# ------------------------------ P3D ------------------------------
+
+def P3D_M2M_vel(particles, mes, kernel, regularisation_radius):
+    """Velocity induced by 3D vortex particles at points mes (n,3).
+    Reference include/cvortex/libcvtx.h:213-220."""
+    p, keep = _ptrs(particles, P3D_DTYPE)
+    mes = np.ascontiguousarray(mes, dtype=np.float32).reshape(-1, 3)
+    out = np.empty((mes.shape[0], 3), np.float32)
+    k, kp = _kref(kernel)
+    _check(lib().cvtx_hip_P3D_M2M_vel(_vp(p), len(p), _vp(mes), mes.shape[0], _vp(out), kp,
+                                      float(regularisation_radius)), "P3D_M2M_vel")
+    return out
+
+
+def P3D_M2M_dvort(particles, induced, kernel, regularisation_radius):
+    """Vortex stretching on `induced` particles.  libcvtx.h:222-229."""
+    p, k1 = _ptrs(particles, P3D_DTYPE)
+    q, k2 = _ptrs(induced, P3D_DTYPE)
+    out = np.empty((len(q), 3), np.float32)
+    k, kp = _kref(kernel)
+    _check(lib().cvtx_hip_P3D_M2M_dvort(_vp(p), len(p), _vp(q), len(q), _vp(out), kp,
+                                        float(regularisation_radius)), "P3D_M2M_dvort")
+    return out
+
+
+def P3D_M2M_visc_dvort(particles, induced, kernel, regularisation_radius, kinematic_visc):
+    """Viscous (PSE) vorticity exchange on `induced`.  libcvtx.h:231-239."""
+    p, k1 = _ptrs(particles, P3D_DTYPE)
+    q, k2 = _ptrs(induced, P3D_DTYPE)
+    out = np.empty((len(q), 3), np.float32)
+    k, kp = _kref(kernel)
+    _check(lib().cvtx_hip_P3D_M2M_visc_dvort(_vp(p), len(p), _vp(q), len(q), _vp(out), kp,
+                                             float(regularisation_radius), float(kinematic_visc)),
+           "P3D_M2M_visc_dvort")
+    return out
+
+
+def P3D_M2M_fused(particles, induced, kernel, regularisation_radius, kinematic_visc=0.0,
+                  vel=True, dvort=True, visc=False):
+    """vel, dvort and/or visc_dvort in one pass (None for outputs not asked)."""
+    p, k1 = _ptrs(particles, P3D_DTYPE)
+    q, k2 = _ptrs(induced, P3D_DTYPE)
+    n = len(q)
+    outs = [np.empty((n, 3), np.float32) if f else None for f in (vel, dvort, visc)]
+    k, kp = _kref(kernel)
+    _check(lib().cvtx_hip_P3D_M2M_fused(_vp(p), len(p), _vp(q), n, *(_vp(o) if o is not None
+                                                                     else None for o in outs),
+                                        kp, float(regularisation_radius), float(kinematic_visc)),
+           "P3D_M2M_fused")
+    return tuple(outs)
+
+
+# ------------------------------ F3D ------------------------------
+
+def F3D_M2M_vel(filaments, mes):
+    """Velocity induced by straight vortex filaments.  libcvtx.h:285-290."""
+    f, keep = _ptrs(filaments, F3D_DTYPE)
+    mes = np.ascontiguousarray(mes, dtype=np.float32).reshape(-1, 3)
+    out = np.empty((mes.shape[0], 3), np.float32)
+    _check(lib().cvtx_hip_F3D_M2M_vel(_vp(f), len(f), _vp(mes), mes.shape[0], _vp(out)),
+           "F3D_M2M_vel")
+    return out
+
+
+def F3D_M2M_dvort(filaments, induced):
+    """Vortex stretching by filaments on particles.  libcvtx.h:292-297."""
+    f, k1 = _ptrs(filaments, F3D_DTYPE)
+    q, k2 = _ptrs(induced, P3D_DTYPE)
+    out = np.empty((len(q), 3), np.float32)
+    _check(lib().cvtx_hip_F3D_M2M_dvort(_vp(f), len(f), _vp(q), len(q), _vp(out)),
+           "F3D_M2M_dvort")
+    return out
+
+
+def F3D_M2M_fused(filaments, induced):
+    f, k1 = _ptrs(filaments, F3D_DTYPE)
+    q, k2 = _ptrs(induced, P3D_DTYPE)
+    vel = np.empty((len(q), 3), np.float32)
+    dv = np.empty((len(q), 3), np.float32)
+    _check(lib().cvtx_hip_F3D_M2M_fused(_vp(f), len(f), _vp(q), len(q), _vp(vel), _vp(dv)),
+           "F3D_M2M_fused")
+    return vel, dv
+
+
+# ------------------------------ P2D ------------------------------
+
+def P2D_M2M_vel(particles, mes, kernel, regularisation_radius):
+    """Velocity induced by 2D vortex particles.  libcvtx.h:329-336."""
+    p, keep = _ptrs(particles, P2D_DTYPE)
+    mes = np.ascontiguousarray(mes, dtype=np.float32).reshape(-1, 2)
+    out = np.empty((mes.shape[0], 2), np.float32)
+    k, kp = _kref(kernel)
+    _check(lib().cvtx_hip_P2D_M2M_vel(_vp(p), len(p), _vp(mes), mes.shape[0], _vp(out), kp,
+                                      float(regularisation_radius)), "P2D_M2M_vel")
+    return out
+
+
+def P2D_M2M_visc_dvort(particles, induced, kernel, regularisation_radius, kinematic_visc):
+    """2D PSE vorticity exchange.  libcvtx.h:362-370."""
+    p, k1 = _ptrs(particles, P2D_DTYPE)
+    q, k2 = _ptrs(induced, P2D_DTYPE)
+    out = np.empty(len(q), np.float32)
+    k, kp = _kref(kernel)
+    _check(lib().cvtx_hip_P2D_M2M_visc_dvort(_vp(p), len(p), _vp(q), len(q), _vp(out), kp,
+                                             float(regularisation_radius), float(kinematic_visc)),
+           "P2D_M2M_visc_dvort")
+    return out
+
+
+def P2D_M2M_fused(particles, induced, kernel, regularisation_radius, kinematic_visc):
+    p, k1 = _ptrs(particles, P2D_DTYPE)
+    q, k2 = _ptrs(induced, P2D_DTYPE)
+    vel = np.empty((len(q), 2), np.float32)
+    vs = np.empty(len(q), np.float32)
+    k, kp = _kref(kernel)
+    _check(lib().cvtx_hip_P2D_M2M_fused(_vp(p), len(p), _vp(q), len(q), _vp(vel), _vp(vs), kp,
+                                        float(regularisation_radius), float(kinematic_visc)),
+           "P2D_M2M_fused")
+    return vel, vs
+
+
+__all__ = [n for n in dir() if not n.startswith("_")] + ["REG_ID", "REG_NAMES"]

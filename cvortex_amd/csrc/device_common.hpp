@@ -1,0 +1,161 @@
+// device_common.hpp -- shared device-side definitions for the gfx950 kernels.
+//
+// Pair kernels are written in the reduced form
+//     vel    = -(1/4pi)       * sum_s  Kv(r2) (d x a_s)
+//     dvort  = sign(sigma)/4pi * [ a_t x sum_s Kv a_s  +  sum_s Kh (a_t . (d x a_s)) d ]
+//     visc   = (2 nu / sigma^2) * [ V_t sum_s eta a_s  -  a_t sum_s eta V_s ]
+// with d = x_t - x_s, Kv = g(rho)/r^3 and Kh = (3 g(rho)/r^3 - zeta(rho)/sigma^3)/r^2.
+// This is algebraically the reference's pair arithmetic (reference
+// src/P3D.cpp:51-144) after using d.(a_t x a_s) = -a_t.(d x a_s) and
+// sum_s (a_t x a_s) Kv = a_t x sum_s Kv a_s; the rearrangement lets vel,
+// dvort and visc share one evaluation of d, r2 and the regularisation per
+// pair and moves two cross products out of the pair loop.
+//
+// Each regularisation computes (Kv, Kh, eta) up to per-call constant factors
+// that the epilogue applies in FP64 (Scales below).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cvtx {
+
+enum Reg : int { REG_SINGULAR = 0, REG_WINCKELMANS = 1, REG_PLANETARY = 2, REG_GAUSSIAN = 3 };
+enum OutMask : int { OUT_VEL = 1, OUT_DVORT = 2, OUT_VISC = 4 };
+enum AccMode : int { ACC_PLAIN = 0, ACC_COMP = 1 };
+
+// Gaussian near-field table: cubic pieces of the reduced kernels
+//   G(rho) = g(rho)/rho^3,  H(rho) = (3 G(rho) - zeta(rho)) / rho^2
+// on rho in [0, GTAB_RHO_MAX) with GTAB_PER_UNIT pieces per unit of rho.
+// Piece i covers rho = (i + t)/GTAB_PER_UNIT, t in [0,1):
+//   G = ((g3 t + g2) t + g1) t + g0, same for H; stored as two float4.
+constexpr int GTAB_PER_UNIT = 16;
+constexpr int GTAB_RHO_MAX = 6;
+constexpr int GTAB_N = GTAB_PER_UNIT * GTAB_RHO_MAX;  // 96 pieces
+
+// Per-call constants for the regularisations (all FP32, set by the host).
+struct RegConsts {
+  float s2;        // sigma^2
+  float inv_s2;    // 1/sigma^2
+  float c1;        // gaussian: -log2(e)/2 | 2D winckelmans eta: 4 log2(e)
+  float c2;        // gaussian: sqrt(2/pi)
+};
+
+// ---- compensated (TwoSum) accumulation ----------------------------------
+// Knuth's TwoSum: hi + lo carries the running sum exactly up to the final
+// rounding of lo.  Six FP32 operations per update; applied once per source
+// tile per accumulator, so its cost is amortised over the tile.
+__device__ __forceinline__ void two_sum_add(float& hi, float& lo, float x) {
+  float s = hi + x;
+  float bp = s - hi;
+  float err = (hi - (s - bp)) + (x - bp);
+  hi = s;
+  lo += err;
+}
+
+// ---- regularisation kernels (reduced form) -------------------------------
+// eval(r2, K, lds_table, Kv, Kh, eta)
+//   Kv: velocity / stretching kernel, zero when r2 == 0 (self and
+//       coincident pairs are excluded, reference src/P3D.cpp:58,94,127)
+//   Kh: gradient kernel; Kh * 0 must be 0 when r2 == 0
+//   eta: PSE kernel, zero when r2 == 0
+template <int REG>
+struct RegKernel;
+
+template <>
+struct RegKernel<REG_SINGULAR> {
+  // Kv = r^-3, Kh = r^-5 (x3 in the epilogue)
+  static __device__ __forceinline__ void eval(float r2, const RegConsts&, const float4*,
+                                              float& Kv, float& Kh, float& eta) {
+    float y = __builtin_amdgcn_rsqf(r2);
+    float y2 = y * y;
+    float kv = y * y2;
+    float kh = kv * y2;
+    bool z = (r2 == 0.f);
+    Kv = z ? 0.f : kv;
+    Kh = z ? 0.f : kh;
+    eta = 0.f;
+  }
+};
+
+template <>
+struct RegKernel<REG_WINCKELMANS> {
+  // Reduced units (q = rho^2, s = q + 1, y = s^-1/2); the epilogue applies
+  // |sigma|^-3 to Kv, 3 |sigma|^-5 to Kh and 52.5 to eta:
+  //   Kv  = G(rho)     = (q + 2.5) y^5
+  //   Kh  = H(rho) / 3 = (q + 3.5) y^7
+  //   eta = y^9
+  static __device__ __forceinline__ void eval(float r2, const RegConsts& k, const float4*,
+                                              float& Kv, float& Kh, float& eta) {
+    float s = __builtin_fmaf(r2, k.inv_s2, 1.f);
+    float y = __builtin_amdgcn_rsqf(s);
+    float y2 = y * y;
+    float y3 = y2 * y;
+    float y5 = y3 * y2;
+    float kv = ((s + 1.5f) * y2) * y3;
+    Kh = ((s + 2.5f) * y2) * y5;
+    float e = (y5 * y2) * y2;
+    bool z = (r2 == 0.f);
+    Kv = z ? 0.f : kv;
+    eta = z ? 0.f : e;
+  }
+};
+
+template <>
+struct RegKernel<REG_PLANETARY> {
+  // Reduced units, q = rho^2.  Inside the core (q < 1): g = rho^3, zeta = 3,
+  // so G = 1 and H = 0; outside: g = 1, zeta = 0, so G = q^-3/2 and
+  // H / 3 = q^-5/2.  The epilogue applies |sigma|^-3 and 3 |sigma|^-5.
+  static __device__ __forceinline__ void eval(float r2, const RegConsts& k, const float4*,
+                                              float& Kv, float& Kh, float& eta) {
+    float q = r2 * k.inv_s2;
+    float y = __builtin_amdgcn_rsqf(q);
+    float y2 = y * y;
+    float kv = y * y2;
+    float kh = kv * y2;
+    bool in = q < 1.f;
+    bool z = (r2 == 0.f);
+    Kv = z ? 0.f : (in ? 1.f : kv);
+    Kh = in ? 0.f : kh;
+    eta = 0.f;
+  }
+};
+
+template <>
+struct RegKernel<REG_GAUSSIAN> {
+  // Works in reduced units: Kv = G(rho), Kh = H(rho) (epilogue scales by
+  // |sigma|^-3 and |sigma|^-5), eta = zeta = sqrt(2/pi) exp(-rho^2/2).
+  // Far field (rho >= 6): erf(rho/sqrt2) = 1 to below 2e-9, so
+  //   G = rho^-3 (1 - rho zeta),  H = (3G - zeta) rho^-2.
+  // Near field (rho < 6): cubic pieces from the LDS table, built on the host
+  // from exact FP64 values (series below rho = 0.5), so the small-rho
+  // behaviour has no cancellation.  Every lane evaluates both; a select
+  // picks one, so a wave never diverges.
+  static __device__ __forceinline__ void eval(float r2, const RegConsts& k, const float4* tab,
+                                              float& Kv, float& Kh, float& eta) {
+    float q = __builtin_fmaf(r2, k.inv_s2, 1e-30f);  // rho^2; the 1e-30 keeps rho finite at r2 = 0
+    float pinv = __builtin_amdgcn_rsqf(q);           // 1/rho
+    float pinv2 = pinv * pinv;
+    float e = __builtin_amdgcn_exp2f(q * k.c1);       // exp(-rho^2/2)
+    float z = k.c2 * e;                               // zeta(rho)
+    float gfar = pinv2 * (pinv - z);
+    float hfar = __builtin_fmaf(3.f, gfar, -z) * pinv2;
+    // near field
+    float t = q * pinv * (float)GTAB_PER_UNIT;        // rho * 16
+    int i = (int)t;
+    i = i < (GTAB_N - 1) ? i : (GTAB_N - 1);
+    float u = t - (float)i;
+    float4 cg = tab[2 * i];
+    float4 ch = tab[2 * i + 1];
+    float gnear = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(cg.w, u, cg.z), u, cg.y), u, cg.x);
+    float hnear = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(ch.w, u, ch.z), u, ch.y), u, ch.x);
+    bool near = q < (float)(GTAB_RHO_MAX * GTAB_RHO_MAX);
+    bool zr = (r2 == 0.f);
+    float g = near ? gnear : gfar;
+    Kv = zr ? 0.f : g;
+    Kh = near ? hnear : hfar;
+    eta = zr ? 0.f : z;
+  }
+};
+
+}  // namespace cvtx

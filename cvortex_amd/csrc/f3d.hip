@@ -1,0 +1,266 @@
+// f3d.hip -- straight singular vortex filament kernels for gfx950.
+//
+// Filaments stream through LDS in tiles; when a tile is staged each
+// filament is expanded once per workgroup into
+//   { a (start), Gamma/(4pi) }  and  { r0 = b - a, 3 Gamma / (4pi |r0|) },
+// so the per-pair work only involves the target-dependent quantities.
+//
+// Per pair (x the target): r1 = x - a, r2 = r1 - r0 (= x - b), c = r0 x r1
+// (= r1 x r2), t = (r0.r1)/|r1| - (r0.r2)/|r2|.
+//   vel:   v += (Gamma/4pi) t / |c|^2 * c              (reference F3D.cpp:34-54)
+//   dvort: B += 3 Gamma/(4pi|r0|) |c| (1/|r1| - 1/|r2|),
+//          S += (Gamma/4pi) t / |c|^2 * r0
+//          dvort = a_t B - S x a_t                       (reference F3D.cpp:56-85)
+// A pair whose terms are not finite (target on a filament's axis or end
+// point) contributes zero, as the reference's GPU kernels do
+// (nbody.cl:637,718).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "kernels.hpp"
+
+namespace cvtx {
+
+int f3d_nacc(int mask) { return ((mask & OUT_VEL) ? 3 : 0) + ((mask & OUT_DVORT) ? 4 : 0); }
+
+struct F3DKParams {
+  const float4* __restrict__ fa;
+  const float4* __restrict__ fb;
+  const float4* __restrict__ tgt_pv;
+  const float4* __restrict__ tgt_a;
+  float* __restrict__ out_vel;
+  float* __restrict__ out_dvort;
+  double* __restrict__ partial;
+  int n_fil, n_tgt, split_len, n_split;
+};
+
+template <int MASK>
+__device__ __forceinline__ void f3d_finish(const double* a, float3 at, const F3DKParams& p, int i) {
+  constexpr int D = (MASK & OUT_VEL) ? 3 : 0;
+  if constexpr ((MASK & OUT_VEL) != 0) {
+    p.out_vel[3 * i + 0] = (float)a[0];
+    p.out_vel[3 * i + 1] = (float)a[1];
+    p.out_vel[3 * i + 2] = (float)a[2];
+  }
+  if constexpr ((MASK & OUT_DVORT) != 0) {
+    const double B = a[D], Sx = a[D + 1], Sy = a[D + 2], Sz = a[D + 3];
+    const double ax = at.x, ay = at.y, az = at.z;
+    p.out_dvort[3 * i + 0] = (float)(ax * B - (Sy * az - Sz * ay));
+    p.out_dvort[3 * i + 1] = (float)(ay * B - (Sz * ax - Sx * az));
+    p.out_dvort[3 * i + 2] = (float)(az * B - (Sx * ay - Sy * ax));
+  }
+}
+
+template <int MASK, int ACC, int T, int WG>
+__global__ __launch_bounds__(WG) void f3d_kernel(F3DKParams p) {
+  constexpr int NA = ((MASK & OUT_VEL) ? 3 : 0) + ((MASK & OUT_DVORT) ? 4 : 0);
+  constexpr int D = (MASK & OUT_VEL) ? 3 : 0;
+  __shared__ float4 s_a[WG];
+  __shared__ float4 s_r0[WG];
+  const int tid = threadIdx.x;
+  const int base = blockIdx.x * (WG * T);
+  float3 xt[T], at[T];
+#pragma unroll
+  for (int k = 0; k < T; ++k) {
+    int i = base + k * WG + tid;
+    i = i < p.n_tgt ? i : p.n_tgt - 1;
+    const float4 q = p.tgt_pv[i];
+    xt[k] = make_float3(q.x, q.y, q.z);
+    if constexpr ((MASK & OUT_DVORT) != 0) {
+      const float4 w = p.tgt_a[i];
+      at[k] = make_float3(w.x, w.y, w.z);
+    } else {
+      at[k] = make_float3(0.f, 0.f, 0.f);
+    }
+  }
+  float acc[T][NA], hi[T][NA], lo[T][NA];
+#pragma unroll
+  for (int k = 0; k < T; ++k)
+#pragma unroll
+    for (int a = 0; a < NA; ++a) { acc[k][a] = 0.f; hi[k][a] = 0.f; lo[k][a] = 0.f; }
+
+  const float k4pi = (float)(1.0 / (4.0 * M_PI));
+  const int s0 = blockIdx.y * p.split_len;
+  const int s1 = min(s0 + p.split_len, p.n_fil);
+  for (int ts = s0; ts < s1; ts += WG) {
+    const int cnt = min(WG, s1 - ts);
+    __syncthreads();
+    if (tid < cnt) {
+      const float4 a = p.fa[ts + tid];
+      const float4 b = p.fb[ts + tid];
+      const float rx = b.x - a.x, ry = b.y - a.y, rz = b.z - a.z;
+      const float g = a.w * k4pi;
+      const float len = __builtin_sqrtf(rx * rx + ry * ry + rz * rz);
+      s_a[tid] = make_float4(a.x, a.y, a.z, g);
+      s_r0[tid] = make_float4(rx, ry, rz, 3.f * g / len);
+    }
+    __syncthreads();
+    if constexpr (ACC == ACC_COMP) {
+#pragma unroll
+      for (int k = 0; k < T; ++k)
+#pragma unroll
+        for (int a = 0; a < NA; ++a) acc[k][a] = 0.f;
+    }
+#pragma unroll 2
+    for (int j = 0; j < cnt; ++j) {
+      const float4 fa = s_a[j];
+      const float4 f0 = s_r0[j];
+#pragma unroll
+      for (int k = 0; k < T; ++k) {
+        const float r1x = xt[k].x - fa.x, r1y = xt[k].y - fa.y, r1z = xt[k].z - fa.z;
+        const float r2x = r1x - f0.x, r2y = r1y - f0.y, r2z = r1z - f0.z;
+        // c = r0 x r1
+        const float cx = __builtin_fmaf(f0.y, r1z, -f0.z * r1y);
+        const float cy = __builtin_fmaf(f0.z, r1x, -f0.x * r1z);
+        const float cz = __builtin_fmaf(f0.x, r1y, -f0.y * r1x);
+        const float cc = __builtin_fmaf(cx, cx, __builtin_fmaf(cy, cy, cz * cz));
+        const float n1 = __builtin_fmaf(r1x, r1x, __builtin_fmaf(r1y, r1y, r1z * r1z));
+        const float n2 = __builtin_fmaf(r2x, r2x, __builtin_fmaf(r2y, r2y, r2z * r2z));
+        const float i1 = __builtin_amdgcn_rsqf(n1);
+        const float i2 = __builtin_amdgcn_rsqf(n2);
+        const float p1 = __builtin_fmaf(f0.x, r1x, __builtin_fmaf(f0.y, r1y, f0.z * r1z));
+        const float p2 = __builtin_fmaf(f0.x, r2x, __builtin_fmaf(f0.y, r2y, f0.z * r2z));
+        const float t = __builtin_fmaf(p1, i1, -p2 * i2);
+        if constexpr ((MASK & OUT_DVORT) != 0) {
+          const float rc = __builtin_amdgcn_rsqf(cc);
+          const float icc = rc * rc;
+          float kk = fa.w * t * icc;
+          float B = f0.w * (cc * rc) * (i1 - i2);
+          const bool ok = (__builtin_fabsf(kk) < __builtin_inff()) && (__builtin_fabsf(B) < __builtin_inff());
+          kk = ok ? kk : 0.f;
+          B = ok ? B : 0.f;
+          if constexpr ((MASK & OUT_VEL) != 0) {
+            acc[k][0] = __builtin_fmaf(kk, cx, acc[k][0]);
+            acc[k][1] = __builtin_fmaf(kk, cy, acc[k][1]);
+            acc[k][2] = __builtin_fmaf(kk, cz, acc[k][2]);
+          }
+          acc[k][D + 0] += B;
+          acc[k][D + 1] = __builtin_fmaf(kk, f0.x, acc[k][D + 1]);
+          acc[k][D + 2] = __builtin_fmaf(kk, f0.y, acc[k][D + 2]);
+          acc[k][D + 3] = __builtin_fmaf(kk, f0.z, acc[k][D + 3]);
+        } else {
+          float kk = fa.w * t * __builtin_amdgcn_rcpf(cc);
+          kk = (__builtin_fabsf(kk) < __builtin_inff()) ? kk : 0.f;
+          acc[k][0] = __builtin_fmaf(kk, cx, acc[k][0]);
+          acc[k][1] = __builtin_fmaf(kk, cy, acc[k][1]);
+          acc[k][2] = __builtin_fmaf(kk, cz, acc[k][2]);
+        }
+      }
+    }
+    if constexpr (ACC == ACC_COMP) {
+#pragma unroll
+      for (int k = 0; k < T; ++k)
+#pragma unroll
+        for (int a = 0; a < NA; ++a) two_sum_add(hi[k][a], lo[k][a], acc[k][a]);
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < T; ++k) {
+    const int i = base + k * WG + tid;
+    if (i >= p.n_tgt) continue;
+    double d[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+      d[a] = (ACC == ACC_COMP) ? (double)hi[k][a] + (double)lo[k][a] : (double)acc[k][a];
+    if (p.n_split == 1) {
+      f3d_finish<MASK>(d, at[k], p, i);
+    } else {
+#pragma unroll
+      for (int a = 0; a < NA; ++a) p.partial[((size_t)blockIdx.y * NA + a) * p.n_tgt + i] = d[a];
+    }
+  }
+}
+
+template <int MASK>
+__global__ __launch_bounds__(256) void f3d_reduce(F3DKParams p) {
+  constexpr int NA = ((MASK & OUT_VEL) ? 3 : 0) + ((MASK & OUT_DVORT) ? 4 : 0);
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= p.n_tgt) return;
+  double d[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) d[a] = 0.0;
+  for (int s = 0; s < p.n_split; ++s)
+#pragma unroll
+    for (int a = 0; a < NA; ++a) d[a] += p.partial[((size_t)s * NA + a) * p.n_tgt + i];
+  float3 at = make_float3(0.f, 0.f, 0.f);
+  if constexpr ((MASK & OUT_DVORT) != 0) {
+    const float4 w = p.tgt_a[i];
+    at = make_float3(w.x, w.y, w.z);
+  }
+  f3d_finish<MASK>(d, at, p, i);
+}
+
+size_t f3d_workspace_bytes(const F3DJob& j, const LaunchCfg& c) {
+  if (c.n_split <= 1) return 0;
+  return (size_t)c.n_split * f3d_nacc(j.mask) * (size_t)j.n_tgt * sizeof(double);
+}
+
+namespace {
+using F3Fn = void (*)(F3DKParams);
+template <int MASK, int ACC>
+F3Fn f3_T(int T) {
+  switch (T) {
+    case 1: return f3d_kernel<MASK, ACC, 1, 256>;
+    case 4: return f3d_kernel<MASK, ACC, 4, 256>;
+    default: return f3d_kernel<MASK, ACC, 2, 256>;
+  }
+}
+template <int MASK>
+F3Fn f3_acc(int acc, int T) {
+  return acc == ACC_PLAIN ? f3_T<MASK, ACC_PLAIN>(T) : f3_T<MASK, ACC_COMP>(T);
+}
+F3Fn f3_pick(int mask, int acc, int T) {
+  switch (mask) {
+    case OUT_VEL: return f3_acc<OUT_VEL>(acc, T);
+    case OUT_DVORT: return f3_acc<OUT_DVORT>(acc, T);
+    case OUT_VEL | OUT_DVORT: return f3_acc<OUT_VEL | OUT_DVORT>(acc, T);
+    default: return nullptr;
+  }
+}
+F3Fn f3_reduce(int mask) {
+  switch (mask) {
+    case OUT_VEL: return f3d_reduce<OUT_VEL>;
+    case OUT_DVORT: return f3d_reduce<OUT_DVORT>;
+    case OUT_VEL | OUT_DVORT: return f3d_reduce<OUT_VEL | OUT_DVORT>;
+    default: return nullptr;
+  }
+}
+}  // namespace
+
+hipError_t launch_f3d(const F3DJob& j, const LaunchCfg& c, hipStream_t s) {
+  if (j.n_tgt <= 0) return hipSuccess;
+  if (j.n_fil < 0 || c.wg != 256) return hipErrorInvalidValue;
+  if ((j.mask & OUT_DVORT) && j.tgt_a == nullptr) return hipErrorInvalidValue;
+  F3Fn k = f3_pick(j.mask, j.acc, c.T);
+  if (!k) return hipErrorInvalidValue;
+  F3DKParams p{};
+  p.fa = j.fa;
+  p.fb = j.fb;
+  p.tgt_pv = j.tgt_pv;
+  p.tgt_a = j.tgt_a;
+  p.out_vel = j.out_vel;
+  p.out_dvort = j.out_dvort;
+  p.partial = j.partial;
+  p.n_fil = j.n_fil;
+  p.n_tgt = j.n_tgt;
+  int n_split = c.n_split, split_len = c.split_len;
+  if (n_split <= 1 || j.n_fil == 0) {
+    n_split = 1;
+    split_len = j.n_fil > 0 ? j.n_fil : 1;
+  } else {
+    if (split_len <= 0 || split_len % c.wg != 0) return hipErrorInvalidValue;
+    if ((long long)split_len * n_split < j.n_fil || j.partial == nullptr) return hipErrorInvalidValue;
+  }
+  p.n_split = n_split;
+  p.split_len = split_len;
+  const int per = c.wg * c.T;
+  hipLaunchKernelGGL(k, dim3((j.n_tgt + per - 1) / per, n_split), dim3(c.wg), 0, s, p);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || n_split == 1) return e;
+  hipLaunchKernelGGL(f3_reduce(j.mask), dim3((j.n_tgt + 255) / 256), dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+}  // namespace cvtx

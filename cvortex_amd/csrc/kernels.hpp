@@ -1,0 +1,108 @@
+// kernels.hpp -- host-visible launch interface of the gfx950 pair kernels.
+//
+// All buffers are device pointers in structure-of-arrays form:
+//   P3D particles: pv = float4 {x, y, z, volume}, a = float4 {ax, ay, az, 0}
+//   P2D particles: float4 {x, y, vorticity, area}
+//   F3D filaments: fa = float4 {start x, y, z, strength}, fb = float4 {end x, y, z, 0}
+//   points (measurement positions): float4 {x, y, z, 0} or {x, y, 0, 0}
+// Outputs are float[3*n] (P3D/F3D), float[2*n] (P2D vel) or float[n] (P2D visc),
+// i.e. exactly the caller's bsv_V3f / bsv_V2f / float result layout.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "device_common.hpp"
+
+namespace cvtx {
+
+// Launch shape chosen by the tuner.
+struct LaunchCfg {
+  int T = 2;          // targets per lane (1, 2 or 4)
+  int wg = 256;       // workgroup size (= source tile length); 256 only for now
+  int n_split = 1;    // number of source partitions (each a grid row)
+  int split_len = 0;  // sources per partition, a multiple of wg
+};
+
+// FP64 epilogue scale factors.
+struct Scales {
+  double vel = 0, dG = 0, dH = 0, visc = 0;
+};
+
+struct P3DJob {
+  const float4* src_pv = nullptr;
+  const float4* src_a = nullptr;
+  int n_src = 0;
+  const float4* tgt_pv = nullptr;
+  const float4* tgt_a = nullptr;  // may be null when mask == OUT_VEL
+  int n_tgt = 0;
+  float* out_vel = nullptr;
+  float* out_dvort = nullptr;
+  float* out_visc = nullptr;
+  int reg = REG_WINCKELMANS;
+  int mask = OUT_VEL;
+  int acc = ACC_COMP;
+  float sigma = 1.f;
+  float nu = 0.f;
+  const float4* gtab = nullptr;  // Gaussian table (device), 2*GTAB_N float4
+  double* partial = nullptr;     // workspace for n_split > 1
+};
+
+struct P2DJob {
+  const float4* src = nullptr;
+  int n_src = 0;
+  const float4* tgt = nullptr;
+  int n_tgt = 0;
+  float* out_vel = nullptr;   // float[2*n_tgt]
+  float* out_visc = nullptr;  // float[n_tgt]
+  int reg = REG_WINCKELMANS;
+  int mask = OUT_VEL;         // OUT_VEL and/or OUT_VISC
+  int acc = ACC_COMP;
+  float sigma = 1.f;
+  float nu = 0.f;
+  double* partial = nullptr;
+};
+
+struct F3DJob {
+  const float4* fa = nullptr;
+  const float4* fb = nullptr;
+  int n_fil = 0;
+  const float4* tgt_pv = nullptr;
+  const float4* tgt_a = nullptr;  // needed for OUT_DVORT
+  int n_tgt = 0;
+  float* out_vel = nullptr;
+  float* out_dvort = nullptr;
+  int mask = OUT_VEL;             // OUT_VEL and/or OUT_DVORT
+  int acc = ACC_COMP;
+  double* partial = nullptr;
+};
+
+// number of FP64 partial accumulators per target for a mask
+int p3d_nacc(int mask);
+int p2d_nacc(int mask);
+int f3d_nacc(int mask);
+
+// Workspace bytes needed for a launch shape (0 when n_split == 1).
+size_t p3d_workspace_bytes(const P3DJob& j, const LaunchCfg& c);
+size_t p2d_workspace_bytes(const P2DJob& j, const LaunchCfg& c);
+size_t f3d_workspace_bytes(const F3DJob& j, const LaunchCfg& c);
+
+// Default launch shape for a problem size (the tuner may override).
+LaunchCfg default_cfg(int n_src, int n_tgt, int n_cu);
+
+// Enqueue the kernels on `s`.  Return hipSuccess or the launch error.
+// Valid combinations: P3D visc only with Winckelmans or Gaussian.
+hipError_t launch_p3d(const P3DJob& j, const LaunchCfg& c, hipStream_t s);
+hipError_t launch_p2d(const P2DJob& j, const LaunchCfg& c, hipStream_t s);
+hipError_t launch_f3d(const F3DJob& j, const LaunchCfg& c, hipStream_t s);
+
+// Host-side helpers.
+Scales p3d_scales(int reg, float sigma, float nu);
+RegConsts p3d_consts(int reg, float sigma);
+bool p3d_supported(int reg, int mask);
+bool p2d_supported(int reg, int mask);
+
+// Gaussian near-field table (2*GTAB_N float4), computed in FP64 on the host.
+void build_gaussian_table(float* out /* 8*GTAB_N floats */);
+
+}  // namespace cvtx

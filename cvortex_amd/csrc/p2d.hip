@@ -1,0 +1,318 @@
+// p2d.hip -- 2D vortex particle direct-sum kernels for gfx950.
+//
+// Same structure as p3d.hip (targets in registers, LDS source tiles shared by
+// the workgroup, TwoSum or plain accumulation, optional source partitions
+// with an FP64 reduction).  Sources and targets are float4 {x, y, alpha, area}.
+//
+// Reduced sums per target (d = x_t - x_s, q = r^2 / sigma^2):
+//   vel  = (1/2pi) * scale_K * ( sum K a_s d_y, -sum K a_s d_x )
+//   visc = (2 nu / sigma^2) * scale_eta * ( A_t sum eta a_s - a_t sum eta A_s )
+// with K = g_2D(rho)/r^2 in the units below.  Reference semantics:
+// src/P2D.cpp:49-69 (vel pair), :99-162 (M2M), :167-276 (visc);
+// src/VortFunc.cpp:78-80,117-131,150-153,191-199 (g_2D, eta_2D).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "kernels.hpp"
+
+namespace cvtx {
+
+int p2d_nacc(int mask) { return ((mask & OUT_VEL) ? 2 : 0) + ((mask & OUT_VISC) ? 2 : 0); }
+
+template <int REG>
+struct Reg2D;
+
+// singular: g = 1 -> K = 1/r^2 (dimensional)
+template <>
+struct Reg2D<REG_SINGULAR> {
+  static __device__ __forceinline__ void eval(float r2, const RegConsts&, float& K, float& eta) {
+    float k = __builtin_amdgcn_rcpf(r2);
+    K = (r2 == 0.f) ? 0.f : k;
+    eta = 0.f;
+  }
+};
+
+// Winckelmans: a = q + 1, y = 1/a
+//   g_2D / r^2 = (q + 2) / ((q + 1)^2 sigma^2)  -> K = (a + 1) y^2  (x 1/sigma^2)
+//   eta_2D = 24 exp(4 / a^3) / a^4  (reference VortFunc.cpp:124-131, kept as is)
+//            -> eta = exp(4 y^3) y^4  (x 24)
+template <>
+struct Reg2D<REG_WINCKELMANS> {
+  static __device__ __forceinline__ void eval(float r2, const RegConsts& k, float& K, float& eta) {
+    float a = __builtin_fmaf(r2, k.inv_s2, 1.f);
+    float y = __builtin_amdgcn_rcpf(a);
+    float y2 = y * y;
+    K = (a + 1.f) * y2;  // finite at r2 = 0 and multiplied by d = 0 there
+    float e = __builtin_amdgcn_exp2f((y2 * y) * k.c1);
+    float et = e * (y2 * y2);
+    eta = (r2 == 0.f) ? 0.f : et;
+  }
+};
+
+// planetary: g = rho^2 inside the core, 1 outside -> K = 1 or 1/q (x 1/sigma^2)
+template <>
+struct Reg2D<REG_PLANETARY> {
+  static __device__ __forceinline__ void eval(float r2, const RegConsts& k, float& K, float& eta) {
+    float q = r2 * k.inv_s2;
+    float iq = __builtin_amdgcn_rcpf(q);
+    K = (q < 1.f) ? 1.f : iq;
+    eta = 0.f;
+  }
+};
+
+// Gaussian: g = 1 - exp(-q/2) -> K = (1 - exp(-q/2)) / q (x 1/sigma^2); for
+// q < 1 the series 1/2 sum_m (-q/2)^m / (m+1)! avoids the cancellation.
+// eta_2D = exp(-q/2).
+template <>
+struct Reg2D<REG_GAUSSIAN> {
+  static __device__ __forceinline__ void eval(float r2, const RegConsts& k, float& K, float& eta) {
+    float q = r2 * k.inv_s2;
+    float e = __builtin_amdgcn_exp2f(q * k.c1);
+    float far = (1.f - e) * __builtin_amdgcn_rcpf(q);
+    float x = -0.5f * q;
+    // 1/2 * sum_{m=0}^{8} x^m / (m+1)!
+    float s = 1.f / 362880.f;
+    s = __builtin_fmaf(s, x, 1.f / 40320.f);
+    s = __builtin_fmaf(s, x, 1.f / 5040.f);
+    s = __builtin_fmaf(s, x, 1.f / 720.f);
+    s = __builtin_fmaf(s, x, 1.f / 120.f);
+    s = __builtin_fmaf(s, x, 1.f / 24.f);
+    s = __builtin_fmaf(s, x, 1.f / 6.f);
+    s = __builtin_fmaf(s, x, 0.5f);
+    s = __builtin_fmaf(s, x, 1.f);
+    K = (q < 1.f) ? 0.5f * s : far;
+    eta = (r2 == 0.f) ? 0.f : e;
+  }
+};
+
+struct P2DKParams {
+  const float4* __restrict__ src;
+  const float4* __restrict__ tgt;
+  float* __restrict__ out_vel;
+  float* __restrict__ out_visc;
+  double* __restrict__ partial;
+  int n_src, n_tgt, split_len, n_split;
+  RegConsts rc;
+  double sc_vel, sc_visc;
+};
+
+template <int MASK>
+__device__ __forceinline__ void p2d_finish(const double* a, float4 t, const P2DKParams& p, int i) {
+  constexpr int V = 0;
+  constexpr int E = (MASK & OUT_VEL) ? 2 : 0;
+  if constexpr ((MASK & OUT_VEL) != 0) {
+    p.out_vel[2 * i + 0] = (float)(p.sc_vel * a[V + 0]);
+    p.out_vel[2 * i + 1] = (float)(-p.sc_vel * a[V + 1]);
+  }
+  if constexpr ((MASK & OUT_VISC) != 0) {
+    p.out_visc[i] = (float)(p.sc_visc * ((double)t.w * a[E + 0] - (double)t.z * a[E + 1]));
+  }
+}
+
+template <int REG, int MASK, int ACC, int T, int WG>
+__global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
+  constexpr int NA = ((MASK & OUT_VEL) ? 2 : 0) + ((MASK & OUT_VISC) ? 2 : 0);
+  constexpr int E = (MASK & OUT_VEL) ? 2 : 0;
+  __shared__ float4 s_src[WG];
+  const int tid = threadIdx.x;
+  const int base = blockIdx.x * (WG * T);
+  float4 xt[T];
+#pragma unroll
+  for (int k = 0; k < T; ++k) {
+    int i = base + k * WG + tid;
+    i = i < p.n_tgt ? i : p.n_tgt - 1;
+    xt[k] = p.tgt[i];
+  }
+  float acc[T][NA], hi[T][NA], lo[T][NA];
+#pragma unroll
+  for (int k = 0; k < T; ++k)
+#pragma unroll
+    for (int a = 0; a < NA; ++a) { acc[k][a] = 0.f; hi[k][a] = 0.f; lo[k][a] = 0.f; }
+
+  const int s0 = blockIdx.y * p.split_len;
+  const int s1 = min(s0 + p.split_len, p.n_src);
+  for (int ts = s0; ts < s1; ts += WG) {
+    const int cnt = min(WG, s1 - ts);
+    __syncthreads();
+    if (tid < cnt) s_src[tid] = p.src[ts + tid];
+    __syncthreads();
+    if constexpr (ACC == ACC_COMP) {
+#pragma unroll
+      for (int k = 0; k < T; ++k)
+#pragma unroll
+        for (int a = 0; a < NA; ++a) acc[k][a] = 0.f;
+    }
+#pragma unroll 4
+    for (int j = 0; j < cnt; ++j) {
+      const float4 s = s_src[j];
+#pragma unroll
+      for (int k = 0; k < T; ++k) {
+        const float dx = xt[k].x - s.x;
+        const float dy = xt[k].y - s.y;
+        const float r2 = __builtin_fmaf(dx, dx, dy * dy);
+        float K, eta;
+        Reg2D<REG>::eval(r2, p.rc, K, eta);
+        if constexpr ((MASK & OUT_VEL) != 0) {
+          const float ka = K * s.z;
+          acc[k][0] = __builtin_fmaf(ka, dy, acc[k][0]);
+          acc[k][1] = __builtin_fmaf(ka, dx, acc[k][1]);
+        }
+        if constexpr ((MASK & OUT_VISC) != 0) {
+          acc[k][E + 0] = __builtin_fmaf(eta, s.z, acc[k][E + 0]);
+          acc[k][E + 1] = __builtin_fmaf(eta, s.w, acc[k][E + 1]);
+        }
+      }
+    }
+    if constexpr (ACC == ACC_COMP) {
+#pragma unroll
+      for (int k = 0; k < T; ++k)
+#pragma unroll
+        for (int a = 0; a < NA; ++a) two_sum_add(hi[k][a], lo[k][a], acc[k][a]);
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < T; ++k) {
+    const int i = base + k * WG + tid;
+    if (i >= p.n_tgt) continue;
+    double d[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+      d[a] = (ACC == ACC_COMP) ? (double)hi[k][a] + (double)lo[k][a] : (double)acc[k][a];
+    if (p.n_split == 1) {
+      p2d_finish<MASK>(d, xt[k], p, i);
+    } else {
+#pragma unroll
+      for (int a = 0; a < NA; ++a) p.partial[((size_t)blockIdx.y * NA + a) * p.n_tgt + i] = d[a];
+    }
+  }
+}
+
+template <int MASK>
+__global__ __launch_bounds__(256) void p2d_reduce(P2DKParams p) {
+  constexpr int NA = ((MASK & OUT_VEL) ? 2 : 0) + ((MASK & OUT_VISC) ? 2 : 0);
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= p.n_tgt) return;
+  double d[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) d[a] = 0.0;
+  for (int s = 0; s < p.n_split; ++s)
+#pragma unroll
+    for (int a = 0; a < NA; ++a) d[a] += p.partial[((size_t)s * NA + a) * p.n_tgt + i];
+  p2d_finish<MASK>(d, p.tgt[i], p, i);
+}
+
+bool p2d_supported(int reg, int mask) {
+  if (reg < REG_SINGULAR || reg > REG_GAUSSIAN) return false;
+  if (mask != OUT_VEL && mask != OUT_VISC && mask != (OUT_VEL | OUT_VISC)) return false;
+  if ((mask & OUT_VISC) && (reg == REG_SINGULAR || reg == REG_PLANETARY)) return false;
+  return true;
+}
+
+size_t p2d_workspace_bytes(const P2DJob& j, const LaunchCfg& c) {
+  if (c.n_split <= 1) return 0;
+  return (size_t)c.n_split * p2d_nacc(j.mask) * (size_t)j.n_tgt * sizeof(double);
+}
+
+namespace {
+using K2Fn = void (*)(P2DKParams);
+
+template <int REG, int MASK, int ACC>
+K2Fn p2_T(int T) {
+  switch (T) {
+    case 1: return p2d_kernel<REG, MASK, ACC, 1, 256>;
+    case 4: return p2d_kernel<REG, MASK, ACC, 4, 256>;
+    default: return p2d_kernel<REG, MASK, ACC, 2, 256>;
+  }
+}
+template <int REG, int MASK>
+K2Fn p2_acc(int acc, int T) {
+  return acc == ACC_PLAIN ? p2_T<REG, MASK, ACC_PLAIN>(T) : p2_T<REG, MASK, ACC_COMP>(T);
+}
+template <int REG>
+K2Fn p2_mask_inviscid(int mask, int acc, int T) {
+  return mask == OUT_VEL ? p2_acc<REG, OUT_VEL>(acc, T) : nullptr;
+}
+template <int REG>
+K2Fn p2_mask_viscous(int mask, int acc, int T) {
+  switch (mask) {
+    case OUT_VEL: return p2_acc<REG, OUT_VEL>(acc, T);
+    case OUT_VISC: return p2_acc<REG, OUT_VISC>(acc, T);
+    case OUT_VEL | OUT_VISC: return p2_acc<REG, OUT_VEL | OUT_VISC>(acc, T);
+    default: return nullptr;
+  }
+}
+K2Fn p2_pick(int reg, int mask, int acc, int T) {
+  switch (reg) {
+    case REG_SINGULAR: return p2_mask_inviscid<REG_SINGULAR>(mask, acc, T);
+    case REG_PLANETARY: return p2_mask_inviscid<REG_PLANETARY>(mask, acc, T);
+    case REG_WINCKELMANS: return p2_mask_viscous<REG_WINCKELMANS>(mask, acc, T);
+    case REG_GAUSSIAN: return p2_mask_viscous<REG_GAUSSIAN>(mask, acc, T);
+    default: return nullptr;
+  }
+}
+K2Fn p2_reduce(int mask) {
+  switch (mask) {
+    case OUT_VEL: return p2d_reduce<OUT_VEL>;
+    case OUT_VISC: return p2d_reduce<OUT_VISC>;
+    case OUT_VEL | OUT_VISC: return p2d_reduce<OUT_VEL | OUT_VISC>;
+    default: return nullptr;
+  }
+}
+}  // namespace
+
+hipError_t launch_p2d(const P2DJob& j, const LaunchCfg& c, hipStream_t s) {
+  if (j.n_tgt <= 0) return hipSuccess;
+  if (!p2d_supported(j.reg, j.mask) || j.n_src < 0 || c.wg != 256) return hipErrorInvalidValue;
+  K2Fn k = p2_pick(j.reg, j.mask, j.acc, c.T);
+  if (!k) return hipErrorInvalidValue;
+  P2DKParams p{};
+  p.src = j.src;
+  p.tgt = j.tgt;
+  p.out_vel = j.out_vel;
+  p.out_visc = j.out_visc;
+  p.partial = j.partial;
+  p.n_src = j.n_src;
+  p.n_tgt = j.n_tgt;
+  const double sa = std::fabs((double)j.sigma);
+  p.rc.s2 = (float)(sa * sa);
+  p.rc.inv_s2 = (float)(1.0 / (sa * sa));
+  const double k2pi = 1.0 / (2.0 * M_PI);
+  switch (j.reg) {
+    case REG_SINGULAR:
+      p.sc_vel = k2pi;
+      break;
+    case REG_WINCKELMANS:
+      p.rc.c1 = (float)(4.0 / std::log(2.0));
+      p.sc_vel = k2pi / (sa * sa);
+      p.sc_visc = 2.0 * (double)j.nu / (sa * sa) * 24.0;
+      break;
+    case REG_PLANETARY:
+      p.sc_vel = k2pi / (sa * sa);
+      break;
+    default:
+      p.rc.c1 = (float)(-0.5 / std::log(2.0));
+      p.sc_vel = k2pi / (sa * sa);
+      p.sc_visc = 2.0 * (double)j.nu / (sa * sa);
+      break;
+  }
+  int n_split = c.n_split, split_len = c.split_len;
+  if (n_split <= 1 || j.n_src == 0) {
+    n_split = 1;
+    split_len = j.n_src > 0 ? j.n_src : 1;
+  } else {
+    if (split_len <= 0 || split_len % c.wg != 0) return hipErrorInvalidValue;
+    if ((long long)split_len * n_split < j.n_src || j.partial == nullptr) return hipErrorInvalidValue;
+  }
+  p.n_split = n_split;
+  p.split_len = split_len;
+  const int per = c.wg * c.T;
+  hipLaunchKernelGGL(k, dim3((j.n_tgt + per - 1) / per, n_split), dim3(c.wg), 0, s, p);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || n_split == 1) return e;
+  hipLaunchKernelGGL(p2_reduce(j.mask), dim3((j.n_tgt + 255) / 256), dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+}  // namespace cvtx

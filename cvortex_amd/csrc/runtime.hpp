@@ -1,0 +1,110 @@
+// runtime.hpp -- device registry, per-device streams and buffers, host gather.
+//
+// Replaces the role of the reference's OpenCL device runtime
+// (src/opencl_acc.cpp, src/OclPlatformState.cpp, src/OclDeviceState.cpp):
+// an accelerator is a HIP device; cvtx_initialise enables the first one;
+// each device gets one stream (default priority) and grow-only device and
+// pinned host buffers that are reused across calls (the reference allocated
+// and freed OpenCL buffers per call and per 256-source chunk,
+// src/ocl_P3D.cpp:193,260-271,289).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace cvtx {
+
+enum Slot : int {
+  SLOT_SRC0 = 0,
+  SLOT_SRC1,
+  SLOT_TGT0,
+  SLOT_TGT1,
+  SLOT_OUT0,
+  SLOT_OUT1,
+  SLOT_OUT2,
+  SLOT_WORK,
+  SLOT_COUNT
+};
+
+struct Buffer {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct Device {
+  int id = -1;
+  std::string name;
+  bool enabled = false;
+  bool ready = false;
+  int n_cu = 0;
+  hipStream_t stream = nullptr;
+  float4* gtab = nullptr;  // Gaussian near-field table
+  Buffer dev[SLOT_COUNT];
+  Buffer host[SLOT_COUNT];  // pinned
+};
+
+struct Runtime {
+  bool initialised = false;
+  std::vector<Device> devices;
+  std::string info;
+  std::string last_error;
+  int acc_mode = ACC_COMP;
+  std::recursive_mutex mu;
+};
+
+Runtime& runtime();
+
+// Idempotent initialisation (device enumeration, device 0 enabled).
+void initialise_locked(Runtime& rt);
+void finalise_locked(Runtime& rt);
+
+// First enabled device, made ready (stream, table).  nullptr and
+// last_error set when there is none or setup fails.
+Device* active_device(Runtime& rt);
+
+// Grow-only buffers.  nullptr (and last_error) on allocation failure.
+void* device_buffer(Runtime& rt, Device& d, int slot, size_t bytes);
+void* host_buffer(Runtime& rt, Device& d, int slot, size_t bytes);
+
+void set_error(Runtime& rt, const std::string& msg);
+bool check_hip(Runtime& rt, hipError_t e, const char* what);
+
+// Run body(begin, end) over [0, n) on up to 16 host threads.
+template <class F>
+void parallel_for(long n, F body);
+
+// Regularisation id from cl_kernel_name_ext, -1 if unknown.
+int reg_from_name(const char* name);
+
+}  // namespace cvtx
+
+#include <thread>
+
+namespace cvtx {
+template <class F>
+void parallel_for(long n, F body) {
+  if (n <= 0) return;
+  unsigned hw = std::thread::hardware_concurrency();
+  long nt = hw == 0 ? 1 : (long)(hw > 16 ? 16 : hw);
+  if (n < 65536 || nt <= 1) {
+    body(0L, n);
+    return;
+  }
+  if (nt > n / 16384) nt = n / 16384;
+  if (nt < 1) nt = 1;
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  const long chunk = (n + nt - 1) / nt;
+  for (long t = 0; t < nt; ++t) {
+    const long b = t * chunk, e = b + chunk < n ? b + chunk : n;
+    if (b >= e) break;
+    th.emplace_back([=]() { body(b, e); });
+  }
+  for (auto& x : th) x.join();
+}
+}  // namespace cvtx

@@ -1,0 +1,207 @@
+// tune.cpp -- launch-shape cache and autotuner.
+//
+// The autotuner times candidate launch shapes (targets per lane T, number of
+// source partitions) for a caller's device-resident problem with HIP events
+// and records the fastest in a cache keyed by
+//   (kind, regularisation, outputs, accumulation, log2 n_src, log2 n_tgt).
+// choose_cfg() consults the cache before falling back to default_cfg().
+// The cache can be saved to and loaded from a small text file, one entry
+// per line: "kind reg mask acc bucket_src bucket_tgt T n_split ms".
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include <cvortex_hip.h>
+
+#include "runtime.hpp"
+#include "tune.hpp"
+
+namespace cvtx {
+
+namespace {
+
+using Key = std::tuple<int, int, int, int, int, int>;
+struct Entry {
+  int T;
+  int n_split;
+  float ms;
+};
+
+std::map<Key, Entry>& cache() {
+  static std::map<Key, Entry> c;
+  return c;
+}
+
+LaunchCfg shape(int T, int n_split, int n_src) {
+  LaunchCfg c;
+  c.T = T;
+  c.wg = 256;
+  const long tiles = ((long)n_src + 255) / 256;
+  if (n_split <= 1 || tiles <= 1) return c;
+  const long per = (tiles + n_split - 1) / n_split;
+  c.split_len = (int)(per * 256);
+  c.n_split = (int)((n_src + c.split_len - 1) / c.split_len);
+  if (c.n_split <= 1) {
+    c.n_split = 1;
+    c.split_len = 0;
+  }
+  return c;
+}
+
+}  // namespace
+
+int size_bucket(int n) {
+  int b = 0;
+  while (n > 1) {
+    n >>= 1;
+    ++b;
+  }
+  return b;
+}
+
+LaunchCfg choose_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt, int n_cu) {
+  auto it = cache().find(Key{kind, reg, mask, acc, size_bucket(n_src), size_bucket(n_tgt)});
+  if (it != cache().end()) return shape(it->second.T, it->second.n_split, n_src);
+  return default_cfg(n_src, n_tgt, n_cu);
+}
+
+void record_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt, int T, int n_split) {
+  cache()[Key{kind, reg, mask, acc, size_bucket(n_src), size_bucket(n_tgt)}] = Entry{T, n_split, 0.f};
+}
+
+}  // namespace cvtx
+
+using namespace cvtx;
+
+namespace {
+
+// Time `reps` launches of one shape; returns ms per launch or -1 on error.
+template <class Launch>
+float time_shape(hipStream_t s, int reps, Launch launch) {
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess) return -1.f;
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return -1.f;
+  }
+  float ms = -1.f;
+  if (launch() == hipSuccess && hipEventRecord(a, s) == hipSuccess) {
+    bool ok = true;
+    for (int r = 0; r < reps && ok; ++r) ok = launch() == hipSuccess;
+    if (ok && hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, a, b) == hipSuccess) ms = t / (float)reps;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms;
+}
+
+std::vector<LaunchCfg> candidates(int n_src, int n_tgt, int n_cu) {
+  std::vector<LaunchCfg> out;
+  const long tiles = ((long)n_src + 255) / 256;
+  for (int T : {1, 2, 4}) {
+    const long blocks = ((long)n_tgt + 256L * T - 1) / (256L * T);
+    if (T > 1 && blocks < 8) continue;
+    for (int k = 1; k <= 256; k *= 2) {
+      if (k > 1 && k > tiles) break;
+      if (blocks * k > 64L * (n_cu > 0 ? n_cu : 256)) break;
+      out.push_back(shape(T, k, n_src));
+    }
+  }
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Autotune the launch shape of a device-resident P3D problem; the winner is
+// recorded in the shape cache for the problem's size bucket.  Returns 0 and
+// the winning shape, or -1.
+int cvtx_hip_autotune_p3d(const cvtx_hip_p3d_problem* p, int reps, int* best_T, int* best_split,
+                          float* best_ms) {
+  Runtime& rt = runtime();
+  std::lock_guard<std::recursive_mutex> lk(rt.mu);
+  if (!p || reps < 1) return -1;
+  Device* d = active_device(rt);
+  if (!d) return -1;
+  P3DJob j;
+  j.src_pv = (const float4*)p->src_pv;
+  j.src_a = (const float4*)p->src_a;
+  j.n_src = p->n_src;
+  j.tgt_pv = (const float4*)p->tgt_pv;
+  j.tgt_a = (const float4*)p->tgt_a;
+  j.n_tgt = p->n_tgt;
+  j.out_vel = p->vel;
+  j.out_dvort = p->dvort;
+  j.out_visc = p->visc;
+  j.reg = p->regularisation;
+  j.mask = (p->vel ? OUT_VEL : 0) | (p->dvort ? OUT_DVORT : 0) | (p->visc ? OUT_VISC : 0);
+  j.acc = p->accumulate;
+  j.sigma = p->sigma;
+  j.nu = p->nu;
+  j.gtab = d->gtab;
+  if (!p3d_supported(j.reg, j.mask) || j.n_tgt <= 0 || j.n_src < 0) return -1;
+  float best = -1.f;
+  LaunchCfg win;
+  for (const LaunchCfg& c : candidates(j.n_src, j.n_tgt, d->n_cu)) {
+    P3DJob jj = j;
+    const size_t wb = p3d_workspace_bytes(jj, c);
+    if (wb) {
+      jj.partial = (double*)device_buffer(rt, *d, SLOT_WORK, wb);
+      if (!jj.partial) return -1;
+    }
+    const float ms = time_shape(d->stream, reps, [&]() { return launch_p3d(jj, c, d->stream); });
+    if (ms > 0.f && (best < 0.f || ms < best)) {
+      best = ms;
+      win = c;
+    }
+  }
+  if (best < 0.f) return -1;
+  cache()[Key{KIND_P3D, j.reg, j.mask, j.acc, size_bucket(j.n_src), size_bucket(j.n_tgt)}] =
+      Entry{win.T, win.n_split, best};
+  if (best_T) *best_T = win.T;
+  if (best_split) *best_split = win.n_split;
+  if (best_ms) *best_ms = best;
+  return 0;
+}
+
+int cvtx_hip_tune_save(const char* path) {
+  FILE* f = std::fopen(path, "w");
+  if (!f) return -1;
+  for (auto& kv : cache()) {
+    const Key& k = kv.first;
+    std::fprintf(f, "%d %d %d %d %d %d %d %d %.6f\n", std::get<0>(k), std::get<1>(k),
+                 std::get<2>(k), std::get<3>(k), std::get<4>(k), std::get<5>(k), kv.second.T,
+                 kv.second.n_split, kv.second.ms);
+  }
+  std::fclose(f);
+  return 0;
+}
+
+int cvtx_hip_tune_load(const char* path) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) return -1;
+  int kind, reg, mask, acc, bs, bt, T, ns;
+  float ms;
+  int n = 0;
+  while (std::fscanf(f, "%d %d %d %d %d %d %d %d %f", &kind, &reg, &mask, &acc, &bs, &bt, &T, &ns,
+                     &ms) == 9) {
+    if ((T == 1 || T == 2 || T == 4) && ns >= 1) {
+      cache()[Key{kind, reg, mask, acc, bs, bt}] = Entry{T, ns, ms};
+      ++n;
+    }
+  }
+  std::fclose(f);
+  return n;
+}
+
+void cvtx_hip_tune_clear(void) { cache().clear(); }
+
+}  // extern "C"

@@ -1,0 +1,195 @@
+/*
+ * cvortex_hip.h -- HIP (gfx950) accelerator seam and device-resident API of
+ * libcvortex.so.
+ *
+ * 1. The seam.  In the reference library every public M2M function first
+ *    tries an int-returning OpenCL accelerator function and runs its own
+ *    OpenMP CPU path when that returns -1 (reference src/P3D.cpp:352-364,
+ *    src/F3D.cpp:169-178, src/P2D.cpp:150-160).  The cvtx_hip_* functions
+ *    below have the same arguments as the public functions, run on the
+ *    first enabled HIP device, and return 0 on success or -1 on failure
+ *    (no device enabled, unsupported regularisation name, invalid argument,
+ *    HIP error); cvtx_hip_last_error() then describes the failure.  They
+ *    never run a CPU fallback themselves.
+ *
+ * 2. Fused host-array entry points (no reference counterpart): one pass
+ *    over the pairs produces several outputs.
+ *
+ * 3. The device-resident API: structure-of-arrays device buffers and a
+ *    caller stream, for callers that keep particles in GPU memory (time
+ *    stepping, batched clouds, graph capture).  All pointers are device
+ *    pointers; `stream` is a hipStream_t passed as void* (NULL = the
+ *    library's stream for the device).  Calls are asynchronous.
+ *
+ * Regularisation ids follow the reference's kernel names
+ * (cl_kernel_name_ext, reference src/VortFunc.cpp:210,223,236,249).
+ */
+#ifndef CVTX_HIP_H
+#define CVTX_HIP_H
+
+#include <stddef.h>
+
+#include <cvortex/libcvtx.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CVTX_HIP_REG_SINGULAR 0
+#define CVTX_HIP_REG_WINCKELMANS 1
+#define CVTX_HIP_REG_PLANETARY 2
+#define CVTX_HIP_REG_GAUSSIAN 3
+
+#define CVTX_HIP_OUT_VEL 1
+#define CVTX_HIP_OUT_DVORT 2
+#define CVTX_HIP_OUT_VISC 4
+
+#define CVTX_HIP_ACC_PLAIN 0 /* one FP32 running sum per output component */
+#define CVTX_HIP_ACC_COMP 1  /* FP32 tile sums + TwoSum across tiles, FP64 across partitions */
+
+/* ---- 1. seam: replaces opencl_brute_force_* (reference src/ocl_P3D.h:33-68,
+ *         src/ocl_F3D.h, src/ocl_P2D.h) ---- */
+
+/* replaces opencl_brute_force_P3D_M2M_vel, reference src/ocl_P3D.cpp:37-62 */
+int cvtx_hip_P3D_M2M_vel(const cvtx_P3D** array_start, int num_particles,
+                         const bsv_V3f* mes_start, int num_mes, bsv_V3f* result_array,
+                         const cvtx_VortFunc* kernel, float regularisation_radius);
+/* replaces opencl_brute_force_P3D_M2M_dvort, reference src/ocl_P3D.cpp:64-91 */
+int cvtx_hip_P3D_M2M_dvort(const cvtx_P3D** array_start, int num_particles,
+                           const cvtx_P3D** induced_start, int num_induced,
+                           bsv_V3f* result_array, const cvtx_VortFunc* kernel,
+                           float regularisation_radius);
+/* replaces opencl_brute_force_P3D_M2M_visc_dvort, reference src/ocl_P3D.cpp:92-119.
+ * Singular and planetary have no PSE kernel: like the reference
+ * (src/VortFunc.cpp:52-62) the result is zero, with a one-time warning. */
+int cvtx_hip_P3D_M2M_visc_dvort(const cvtx_P3D** array_start, int num_particles,
+                                const cvtx_P3D** induced_start, int num_induced,
+                                bsv_V3f* result_array, const cvtx_VortFunc* kernel,
+                                float regularisation_radius, float kinematic_visc);
+/* replaces opencl_brute_force_F3D_M2M_vel, reference src/ocl_F3D.cpp:36-70 */
+int cvtx_hip_F3D_M2M_vel(const cvtx_F3D** array_start, int num_filaments,
+                         const bsv_V3f* mes_start, int num_mes, bsv_V3f* result_array);
+/* replaces opencl_brute_force_F3D_M2M_dvort, reference src/ocl_F3D.cpp:422-445 */
+int cvtx_hip_F3D_M2M_dvort(const cvtx_F3D** array_start, int num_filaments,
+                           const cvtx_P3D** induced_start, int num_induced,
+                           bsv_V3f* result_array);
+/* replaces opencl_brute_force_P2D_M2M_vel, reference src/ocl_P2D.cpp:37-76 */
+int cvtx_hip_P2D_M2M_vel(const cvtx_P2D** array_start, int num_particles,
+                         const bsv_V2f* mes_start, int num_mes, bsv_V2f* result_array,
+                         const cvtx_VortFunc* kernel, float regularisation_radius);
+/* replaces opencl_brute_force_P2D_M2M_visc_dvort, reference src/ocl_P2D.cpp:78-105 */
+int cvtx_hip_P2D_M2M_visc_dvort(const cvtx_P2D** array_start, int num_particles,
+                                const cvtx_P2D** induced_start, int num_induced,
+                                float* result_array, const cvtx_VortFunc* kernel,
+                                float regularisation_radius, float kinematic_visc);
+
+/* ---- 2. fused host-array entry points (no reference counterpart) ----
+ * Any of the result pointers may be NULL (that output is not computed). */
+int cvtx_hip_P3D_M2M_fused(const cvtx_P3D** array_start, int num_particles,
+                           const cvtx_P3D** induced_start, int num_induced,
+                           bsv_V3f* vel, bsv_V3f* dvort, bsv_V3f* visc_dvort,
+                           const cvtx_VortFunc* kernel, float regularisation_radius,
+                           float kinematic_visc);
+int cvtx_hip_F3D_M2M_fused(const cvtx_F3D** array_start, int num_filaments,
+                           const cvtx_P3D** induced_start, int num_induced,
+                           bsv_V3f* vel, bsv_V3f* dvort);
+int cvtx_hip_P2D_M2M_fused(const cvtx_P2D** array_start, int num_particles,
+                           const cvtx_P2D** induced_start, int num_induced,
+                           bsv_V2f* vel, float* visc_dvort, const cvtx_VortFunc* kernel,
+                           float regularisation_radius, float kinematic_visc);
+
+/* ---- controls (no reference counterpart) ---- */
+const char* cvtx_hip_last_error(void);
+/* Accumulation used by every call that does not name one; returns the
+ * previous mode, or -1 for an invalid mode. Default CVTX_HIP_ACC_COMP. */
+int cvtx_hip_set_accumulation(int mode);
+int cvtx_hip_get_accumulation(void);
+/* The library's stream on the active device (hipStream_t as void*), NULL
+ * when no device is enabled. */
+void* cvtx_hip_stream(void);
+/* Block until all work queued by the library on the active device is done. */
+int cvtx_hip_synchronize(void);
+
+/* ---- 3. device-resident API (no reference counterpart) ---- */
+
+/* P3D particles: pv = float4 {x, y, z, volume}, a = float4 {ax, ay, az, -}.
+ * Targets: tgt_pv (float4) always; tgt_a needed for dvort and visc.
+ * Outputs: float[3*n_tgt] each (NULL = not computed). */
+typedef struct {
+	const float* src_pv;
+	const float* src_a;
+	int n_src;
+	const float* tgt_pv;
+	const float* tgt_a;
+	int n_tgt;
+	float* vel;
+	float* dvort;
+	float* visc;
+	int regularisation; /* CVTX_HIP_REG_* */
+	float sigma;        /* regularisation radius */
+	float nu;           /* kinematic viscosity (visc only) */
+	int accumulate;     /* CVTX_HIP_ACC_* */
+} cvtx_hip_p3d_problem;
+
+/* P2D particles and targets: float4 {x, y, vorticity, area}.
+ * Outputs: vel float[2*n_tgt], visc float[n_tgt] (NULL = not computed). */
+typedef struct {
+	const float* src;
+	int n_src;
+	const float* tgt;
+	int n_tgt;
+	float* vel;
+	float* visc;
+	int regularisation;
+	float sigma;
+	float nu;
+	int accumulate;
+} cvtx_hip_p2d_problem;
+
+/* F3D filaments: fa = float4 {start x, y, z, strength}, fb = float4 {end x, y, z, -}.
+ * Targets as for P3D; tgt_a needed for dvort. */
+typedef struct {
+	const float* fa;
+	const float* fb;
+	int n_fil;
+	const float* tgt_pv;
+	const float* tgt_a;
+	int n_tgt;
+	float* vel;
+	float* dvort;
+	int accumulate;
+} cvtx_hip_f3d_problem;
+
+/* Evaluate with the tuned (or default) launch shape.  A source-partitioned
+ * launch needs workspace; the library keeps a per-device workspace that it
+ * grows on demand (this may allocate device memory on first use of a size). */
+int cvtx_hip_p3d_eval(const cvtx_hip_p3d_problem* p, void* stream);
+int cvtx_hip_p2d_eval(const cvtx_hip_p2d_problem* p, void* stream);
+int cvtx_hip_f3d_eval(const cvtx_hip_f3d_problem* p, void* stream);
+
+/* Same, with an explicit launch shape: targets_per_lane in {1, 2, 4},
+ * n_split >= 1 source partitions (rounded to whole 256-source tiles). */
+int cvtx_hip_p3d_eval_shape(const cvtx_hip_p3d_problem* p, int targets_per_lane, int n_split,
+                            void* stream);
+int cvtx_hip_p2d_eval_shape(const cvtx_hip_p2d_problem* p, int targets_per_lane, int n_split,
+                            void* stream);
+int cvtx_hip_f3d_eval_shape(const cvtx_hip_f3d_problem* p, int targets_per_lane, int n_split,
+                            void* stream);
+
+/* ---- autotuner (no reference counterpart) ----
+ * Times the candidate launch shapes of a device-resident problem on the
+ * active device (the problem's outputs are overwritten) and records the
+ * fastest for the problem's size bucket; later calls of any P3D entry point
+ * in that bucket use it.  Returns 0 and the winner, or -1. */
+int cvtx_hip_autotune_p3d(const cvtx_hip_p3d_problem* p, int reps, int* best_T, int* best_split,
+                          float* best_ms);
+/* Save / load the shape cache as text; load returns the number of entries. */
+int cvtx_hip_tune_save(const char* path);
+int cvtx_hip_tune_load(const char* path);
+void cvtx_hip_tune_clear(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* CVTX_HIP_H */

@@ -1,0 +1,304 @@
+"""CPU oracle for the cvortex direct-sum hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() (as the checker) and bench.py's
+cpu_baseline leg may import this package.  The shipped library never loads
+it.
+
+Wraps liboracle.so (built from oracle/refcpu.c and oracle/fp64.c):
+
+  ref_*   FP32 restatement of the reference's OpenMP CPU path (the
+          "reference result" of every parity test and the CPU baseline);
+  f64_*   double-precision oracle; gauss_mode 0 evaluates the reference's
+          Abramowitz & Stegun Gaussian in FP64, 1 uses exact erf.
+
+See oracle/oracle.h for the pinning status of each function.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from cvortex_amd.types import (F3D_DTYPE, P2D_DTYPE, P3D_DTYPE, REG_ID, V2f, V3f, P3D, F3D, P2D,
+                               as_records, pointer_array)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_D = ctypes.c_double
+
+_SIGS = {
+    "orc_vf_eval": (_F, [_I, _I, _F]),
+    "orc_num_threads": (_I, []),
+    "orc_P3D_S2S_vel": (V3f, [ctypes.POINTER(P3D), V3f, _I, _F]),
+    "orc_P3D_S2S_dvort": (V3f, [ctypes.POINTER(P3D), ctypes.POINTER(P3D), _I, _F]),
+    "orc_P3D_S2S_visc_dvort": (V3f, [ctypes.POINTER(P3D), ctypes.POINTER(P3D), _I, _F, _F]),
+    "orc_P3D_M2M_vel": (None, [_VP, _I, _VP, _I, _VP, _I, _F]),
+    "orc_P3D_M2M_dvort": (None, [_VP, _I, _VP, _I, _VP, _I, _F]),
+    "orc_P3D_M2M_visc_dvort": (None, [_VP, _I, _VP, _I, _VP, _I, _F, _F]),
+    "orc_F3D_S2S_vel": (V3f, [ctypes.POINTER(F3D), V3f]),
+    "orc_F3D_S2S_dvort": (V3f, [ctypes.POINTER(F3D), ctypes.POINTER(P3D)]),
+    "orc_F3D_M2M_vel": (None, [_VP, _I, _VP, _I, _VP]),
+    "orc_F3D_M2M_dvort": (None, [_VP, _I, _VP, _I, _VP]),
+    "orc_P2D_S2S_vel": (V2f, [ctypes.POINTER(P2D), V2f, _I, _F]),
+    "orc_P2D_S2S_visc_dvort": (_F, [ctypes.POINTER(P2D), ctypes.POINTER(P2D), _I, _F, _F]),
+    "orc_P2D_M2M_vel": (None, [_VP, _I, _VP, _I, _VP, _I, _F]),
+    "orc_P2D_M2M_visc_dvort": (None, [_VP, _I, _VP, _I, _VP, _I, _F, _F]),
+    "orc64_vf_eval": (_D, [_I, _I, _D, _I]),
+    "orc64_P3D_M2M": (None, [_VP, _I, _VP, _VP, _I, _I, _D, _D, _I, _VP, _VP, _VP]),
+    "orc64_P3D_M2M_vel_pts": (None, [_VP, _I, _VP, _VP, _I, _I, _D, _I, _VP]),
+    "orc64_F3D_M2M": (None, [_VP, _I, _VP, _VP, _I, _VP, _VP]),
+    "orc64_F3D_M2M_vel_pts": (None, [_VP, _I, _VP, _VP, _I, _VP]),
+    "orc64_P2D_M2M": (None, [_VP, _I, _VP, _VP, _I, _I, _D, _D, _I, _VP, _VP]),
+    "orc64_P2D_M2M_vel_pts": (None, [_VP, _I, _VP, _VP, _I, _I, _D, _I, _VP]),
+}
+
+FN = {"g3": 0, "g2": 1, "zeta3": 2, "eta3": 3, "eta2": 4, "comb_g": 5, "comb_zeta": 6}
+
+
+def build(mutants=False):
+    """Compile liboracle.so (and optionally the planted-mistake variants)."""
+    targets = ["all"] + (["mutants"] if mutants else [])
+    subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
+
+
+def load(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} missing; run oracle.build()")
+    L = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = load()
+    return _lib
+
+
+def _vp(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+def _reg(kernel):
+    return REG_ID[kernel] if isinstance(kernel, str) else int(kernel)
+
+
+class Oracle:
+    """Oracle bound to one shared library (the default one or a mutant)."""
+
+    def __init__(self, L):
+        self.L = L
+
+    # ---------------- regularisations ----------------
+    def vf(self, kernel, fn, rho):
+        return self.L.orc_vf_eval(_reg(kernel), FN[fn], float(rho))
+
+    def vf64(self, kernel, fn, rho, gauss_mode=1):
+        return self.L.orc64_vf_eval(_reg(kernel), FN[fn], float(rho), gauss_mode)
+
+    # ---------------- single pairs ----------------
+    def P3D_S2S_vel(self, p, mes, kernel, sigma):
+        pp = P3D.from_buffer_copy(np.ascontiguousarray(p).tobytes())
+        m = V3f((ctypes.c_float * 3)(*mes))
+        r = self.L.orc_P3D_S2S_vel(ctypes.byref(pp), m, _reg(kernel), float(sigma))
+        return np.array(r.x[:], np.float32)
+
+    def P3D_S2S_dvort(self, p, ind, kernel, sigma):
+        a = P3D.from_buffer_copy(np.ascontiguousarray(p).tobytes())
+        b = P3D.from_buffer_copy(np.ascontiguousarray(ind).tobytes())
+        r = self.L.orc_P3D_S2S_dvort(ctypes.byref(a), ctypes.byref(b), _reg(kernel), float(sigma))
+        return np.array(r.x[:], np.float32)
+
+    def P3D_S2S_visc_dvort(self, p, ind, kernel, sigma, nu):
+        a = P3D.from_buffer_copy(np.ascontiguousarray(p).tobytes())
+        b = P3D.from_buffer_copy(np.ascontiguousarray(ind).tobytes())
+        r = self.L.orc_P3D_S2S_visc_dvort(ctypes.byref(a), ctypes.byref(b), _reg(kernel),
+                                          float(sigma), float(nu))
+        return np.array(r.x[:], np.float32)
+
+    def F3D_S2S_vel(self, f, mes):
+        ff = F3D.from_buffer_copy(np.ascontiguousarray(f).tobytes())
+        m = V3f((ctypes.c_float * 3)(*mes))
+        return np.array(self.L.orc_F3D_S2S_vel(ctypes.byref(ff), m).x[:], np.float32)
+
+    def F3D_S2S_dvort(self, f, ind):
+        ff = F3D.from_buffer_copy(np.ascontiguousarray(f).tobytes())
+        b = P3D.from_buffer_copy(np.ascontiguousarray(ind).tobytes())
+        return np.array(self.L.orc_F3D_S2S_dvort(ctypes.byref(ff), ctypes.byref(b)).x[:], np.float32)
+
+    def P2D_S2S_vel(self, p, mes, kernel, sigma):
+        pp = P2D.from_buffer_copy(np.ascontiguousarray(p).tobytes())
+        m = V2f((ctypes.c_float * 2)(*mes))
+        return np.array(self.L.orc_P2D_S2S_vel(ctypes.byref(pp), m, _reg(kernel), float(sigma)).x[:],
+                        np.float32)
+
+    def P2D_S2S_visc_dvort(self, p, ind, kernel, sigma, nu):
+        a = P2D.from_buffer_copy(np.ascontiguousarray(p).tobytes())
+        b = P2D.from_buffer_copy(np.ascontiguousarray(ind).tobytes())
+        return self.L.orc_P2D_S2S_visc_dvort(ctypes.byref(a), ctypes.byref(b), _reg(kernel),
+                                             float(sigma), float(nu))
+
+    # ---------------- FP32 restatement, many-to-many ----------------
+    def P3D_M2M_vel(self, particles, mes, kernel, sigma):
+        p, keep = pointer_array(as_records(particles, P3D_DTYPE))
+        mes = np.ascontiguousarray(mes, np.float32).reshape(-1, 3)
+        out = np.empty_like(mes)
+        self.L.orc_P3D_M2M_vel(_vp(p), len(p), _vp(mes), len(mes), _vp(out), _reg(kernel),
+                               float(sigma))
+        return out
+
+    def P3D_M2M_dvort(self, particles, induced, kernel, sigma):
+        p, k1 = pointer_array(as_records(particles, P3D_DTYPE))
+        q, k2 = pointer_array(as_records(induced, P3D_DTYPE))
+        out = np.empty((len(q), 3), np.float32)
+        self.L.orc_P3D_M2M_dvort(_vp(p), len(p), _vp(q), len(q), _vp(out), _reg(kernel),
+                                 float(sigma))
+        return out
+
+    def P3D_M2M_visc_dvort(self, particles, induced, kernel, sigma, nu):
+        p, k1 = pointer_array(as_records(particles, P3D_DTYPE))
+        q, k2 = pointer_array(as_records(induced, P3D_DTYPE))
+        out = np.empty((len(q), 3), np.float32)
+        self.L.orc_P3D_M2M_visc_dvort(_vp(p), len(p), _vp(q), len(q), _vp(out), _reg(kernel),
+                                      float(sigma), float(nu))
+        return out
+
+    def F3D_M2M_vel(self, fils, mes):
+        f, keep = pointer_array(as_records(fils, F3D_DTYPE))
+        mes = np.ascontiguousarray(mes, np.float32).reshape(-1, 3)
+        out = np.empty_like(mes)
+        self.L.orc_F3D_M2M_vel(_vp(f), len(f), _vp(mes), len(mes), _vp(out))
+        return out
+
+    def F3D_M2M_dvort(self, fils, induced):
+        f, k1 = pointer_array(as_records(fils, F3D_DTYPE))
+        q, k2 = pointer_array(as_records(induced, P3D_DTYPE))
+        out = np.empty((len(q), 3), np.float32)
+        self.L.orc_F3D_M2M_dvort(_vp(f), len(f), _vp(q), len(q), _vp(out))
+        return out
+
+    def P2D_M2M_vel(self, particles, mes, kernel, sigma):
+        p, keep = pointer_array(as_records(particles, P2D_DTYPE))
+        mes = np.ascontiguousarray(mes, np.float32).reshape(-1, 2)
+        out = np.empty_like(mes)
+        self.L.orc_P2D_M2M_vel(_vp(p), len(p), _vp(mes), len(mes), _vp(out), _reg(kernel),
+                               float(sigma))
+        return out
+
+    def P2D_M2M_visc_dvort(self, particles, induced, kernel, sigma, nu):
+        p, k1 = pointer_array(as_records(particles, P2D_DTYPE))
+        q, k2 = pointer_array(as_records(induced, P2D_DTYPE))
+        out = np.empty(len(q), np.float32)
+        self.L.orc_P2D_M2M_visc_dvort(_vp(p), len(p), _vp(q), len(q), _vp(out), _reg(kernel),
+                                      float(sigma), float(nu))
+        return out
+
+    # ---------------- FP64 oracle ----------------
+    def f64_P3D(self, particles, induced, kernel, sigma, nu=0.0, idx=None, gauss_mode=1,
+                vel=True, dvort=True, visc=False):
+        p, k1 = pointer_array(as_records(particles, P3D_DTYPE))
+        q, k2 = pointer_array(as_records(induced, P3D_DTYPE))
+        idx = np.arange(len(q), dtype=np.int32) if idx is None else np.ascontiguousarray(idx, np.int32)
+        n = len(idx)
+        outs = [np.zeros((n, 3)) if f else None for f in (vel, dvort, visc)]
+        self.L.orc64_P3D_M2M(_vp(p), len(p), _vp(q), _vp(idx), n, _reg(kernel), float(sigma),
+                             float(nu), gauss_mode, *[_vp(o) for o in outs])
+        return tuple(outs)
+
+    def f64_P3D_vel_pts(self, particles, mes, kernel, sigma, idx=None, gauss_mode=1):
+        p, k1 = pointer_array(as_records(particles, P3D_DTYPE))
+        mes = np.ascontiguousarray(mes, np.float32).reshape(-1, 3)
+        idx = np.arange(len(mes), dtype=np.int32) if idx is None else np.ascontiguousarray(idx, np.int32)
+        out = np.zeros((len(idx), 3))
+        self.L.orc64_P3D_M2M_vel_pts(_vp(p), len(p), _vp(mes), _vp(idx), len(idx), _reg(kernel),
+                                     float(sigma), gauss_mode, _vp(out))
+        return out
+
+    def f64_F3D(self, fils, induced, idx=None, vel=True, dvort=True):
+        f, k1 = pointer_array(as_records(fils, F3D_DTYPE))
+        q, k2 = pointer_array(as_records(induced, P3D_DTYPE))
+        idx = np.arange(len(q), dtype=np.int32) if idx is None else np.ascontiguousarray(idx, np.int32)
+        outs = [np.zeros((len(idx), 3)) if x else None for x in (vel, dvort)]
+        self.L.orc64_F3D_M2M(_vp(f), len(f), _vp(q), _vp(idx), len(idx), *[_vp(o) for o in outs])
+        return tuple(outs)
+
+    def f64_F3D_vel_pts(self, fils, mes, idx=None):
+        f, k1 = pointer_array(as_records(fils, F3D_DTYPE))
+        mes = np.ascontiguousarray(mes, np.float32).reshape(-1, 3)
+        idx = np.arange(len(mes), dtype=np.int32) if idx is None else np.ascontiguousarray(idx, np.int32)
+        out = np.zeros((len(idx), 3))
+        self.L.orc64_F3D_M2M_vel_pts(_vp(f), len(f), _vp(mes), _vp(idx), len(idx), _vp(out))
+        return out
+
+    def f64_P2D(self, particles, induced, kernel, sigma, nu=0.0, idx=None, gauss_mode=1,
+                vel=True, visc=False):
+        p, k1 = pointer_array(as_records(particles, P2D_DTYPE))
+        q, k2 = pointer_array(as_records(induced, P2D_DTYPE))
+        idx = np.arange(len(q), dtype=np.int32) if idx is None else np.ascontiguousarray(idx, np.int32)
+        n = len(idx)
+        v = np.zeros((n, 2)) if vel else None
+        e = np.zeros(n) if visc else None
+        self.L.orc64_P2D_M2M(_vp(p), len(p), _vp(q), _vp(idx), n, _reg(kernel), float(sigma),
+                             float(nu), gauss_mode, _vp(v), _vp(e))
+        return v, e
+
+    def f64_P2D_vel_pts(self, particles, mes, kernel, sigma, idx=None, gauss_mode=1):
+        p, k1 = pointer_array(as_records(particles, P2D_DTYPE))
+        mes = np.ascontiguousarray(mes, np.float32).reshape(-1, 2)
+        idx = np.arange(len(mes), dtype=np.int32) if idx is None else np.ascontiguousarray(idx, np.int32)
+        out = np.zeros((len(idx), 2))
+        self.L.orc64_P2D_M2M_vel_pts(_vp(p), len(p), _vp(mes), _vp(idx), len(idx), _reg(kernel),
+                                     float(sigma), gauss_mode, _vp(out))
+        return out
+
+
+def default():
+    """Oracle bound to liboracle.so."""
+    return Oracle(lib())
+
+
+def mutant(k):
+    """Oracle bound to the variant with planted mistake k (see refcpu.c)."""
+    return Oracle(load(os.path.join(HERE, "_mut", f"liboracle_m{k}.so")))
+
+
+# ---------------- error metrics ----------------
+
+def rel_l2(a, b):
+    """||a - b|| / ||b|| over all components."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a - b))
+
+
+def per_target_rel(a, b, floor=0.0):
+    """|a_i - b_i| / |b_i| per target (rows), ignoring rows with |b_i| <= floor."""
+    a = np.asarray(a, np.float64).reshape(len(a), -1)
+    b = np.asarray(b, np.float64).reshape(len(b), -1)
+    nb = np.linalg.norm(b, axis=1)
+    d = np.linalg.norm(a - b, axis=1)
+    m = nb > floor
+    return d[m] / nb[m]
+
+
+def ref_test_metric(a, b):
+    """The reference's own agreement metric (test/testsamecpugpuresultmany.h:
+    100-104): |a - b| / |a + b| per target where |a + b| > 2e-35."""
+    a = np.asarray(a, np.float64).reshape(len(a), -1)
+    b = np.asarray(b, np.float64).reshape(len(b), -1)
+    m = np.linalg.norm(a - b, axis=1)
+    p = np.linalg.norm(a + b, axis=1)
+    keep = p > 2e-35
+    return m[keep] / p[keep]

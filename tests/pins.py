@@ -1,0 +1,444 @@
+"""Checks that pin the CPU oracle to the reference and to mathematics.
+
+Each check takes an `oracle.Oracle` and returns a list of failure messages
+(empty when it passes).  tests/test_oracle_pins.py runs every check on the
+oracle; tests/test_oracle_mutants.py runs them on builds of the oracle with
+one planted mistake each and requires every such build to fail at least one
+check.  Checks are chosen so that each compares the oracle with something
+other than its own formula: the reference's known-answer tests, the
+reference's second (OpenCL) statement of a kernel, textbook closed forms,
+quadrature of the Biot-Savart integral, finite-difference gradients,
+moment conditions of the regularisation kernels, conservation, or the
+independently written FP64 oracle.
+"""
+import math
+
+import numpy as np
+
+from cvortex_amd.types import make_f3d, make_p2d, make_p3d
+from cvortex_amd import workloads as W
+import oracle as orc
+
+REGS = ("singular", "winckelmans", "planetary", "gaussian")
+
+
+def _close(a, b, rtol, atol=0.0):
+    return abs(a - b) <= atol + rtol * abs(b)
+
+
+# ---------------------------------------------------------------------------
+# A. reference known-answer tests, test/testvortfunc.h:36-67
+# ---------------------------------------------------------------------------
+def kat_vortfunc(o):
+    f = []
+    v = lambda k, fn, r: o.vf(k, fn, r)
+    checks = [
+        (v("singular", "g3", 0.1) == 1.0, "singular g(0.1) == 1"),
+        (v("singular", "g3", 3.0) == 1.0, "singular g(3) == 1"),
+        (v("singular", "zeta3", 0.1) == 0.0, "singular zeta(0.1) == 0"),
+        (v("singular", "zeta3", 10.0) == 0.0, "singular zeta(10) == 0"),
+        (v("winckelmans", "g3", 0.0) == 0.0, "winckelmans g(0) == 0"),
+        (abs(v("winckelmans", "g3", 1.0) - 0.61872) < 1e-5, "winckelmans g(1)"),
+        (abs(v("winckelmans", "g3", 10.0) - 0.9998168) < 1e-7, "winckelmans g(10)"),
+        (v("winckelmans", "zeta3", 0.0) == 7.5, "winckelmans zeta(0) == 7.5"),
+        (abs(v("winckelmans", "zeta3", 10.0) - 7.2433e-7) < 1e-11, "winckelmans zeta(10)"),
+        (v("planetary", "g3", 0.0) == 0.0, "planetary g(0)"),
+        (v("planetary", "g3", 1.0) == 1.0, "planetary g(1)"),
+        (v("planetary", "g3", 10.0) == 1.0, "planetary g(10)"),
+        (v("planetary", "zeta3", 0.99) == 3.0, "planetary zeta(0.99)"),
+        (v("planetary", "zeta3", 1.01) == 0.0, "planetary zeta(1.01)"),
+        (abs(v("gaussian", "g3", 0.5) - 0.030859595) < 1e-6, "gaussian g(0.5)"),
+        (abs(v("gaussian", "g3", 1.0) - 0.198748043) < 1e-6, "gaussian g(1)"),
+        (abs(v("gaussian", "g3", 2.0) - 0.738535870) < 1e-6, "gaussian g(2)"),
+        (abs(v("gaussian", "g3", 4.0) - 0.998866015) < 1e-6, "gaussian g(4)"),
+        (abs(v("gaussian", "g3", 6.0) - 0.999999925) < 1e-6, "gaussian g(6)"),
+        (abs(v("gaussian", "g3", 8.0) - 0.999999999) < 1e-6, "gaussian g(8)"),
+        (v("gaussian", "g3", 10.0) == 1.0, "gaussian g(10) == 1"),
+        (abs(v("gaussian", "zeta3", 1.0) - 0.483941449) < 1e-6, "gaussian zeta(1)"),
+        (abs(v("gaussian", "zeta3", 0.5) - 0.70413065) < 1e-6, "gaussian zeta(0.5)"),
+    ]
+    for k in ("winckelmans", "planetary", "gaussian", "singular"):
+        for r in (0.3, 1.7):
+            checks.append((v(k, "comb_g", r) == v(k, "g3", r) and v(k, "comb_zeta", r) == v(k, "zeta3", r),
+                           f"{k} combined_3D == (g_3D, zeta_3D) at {r}"))
+    for ok, msg in checks:
+        if not ok:
+            f.append("KAT " + msg)
+    return f
+
+
+def _quad(fn, a, b, n=4001):
+    """Composite Simpson on [a, b]."""
+    x = np.linspace(a, b, n)
+    y = np.array([fn(t) for t in x], np.float64)
+    h = (b - a) / (n - 1)
+    return h / 3.0 * (y[0] + y[-1] + 4.0 * y[1:-1:2].sum() + 2.0 * y[2:-1:2].sum())
+
+
+# ---------------------------------------------------------------------------
+# B/C. moment conditions and g = int zeta rho^2 (properties every 3D
+# regularisation obeys: unit circulation, and the PSE second moment, which
+# is 3 for both of the reference's eta kernels)
+# ---------------------------------------------------------------------------
+def moments_3d(o):
+    f = []
+    # zeta normalisation: int_0^inf zeta(rho) rho^2 drho = 1
+    for k, top in (("winckelmans", 200.0), ("gaussian", 12.0)):
+        m0 = _quad(lambda r: o.vf(k, "zeta3", r) * r * r, 0.0, top, 20001)
+        tail = 7.5 * (top ** -2) / 2 if k == "winckelmans" else 0.0  # int_top^inf 7.5 r^-5 r^2
+        if not _close(m0 + tail, 1.0, 1e-4):
+            f.append(f"{k}: int zeta rho^2 = {m0 + tail}, expected 1")
+    m0 = _quad(lambda r: o.vf("planetary", "zeta3", r) * r * r, 0.0, 0.999999, 4001)
+    if not _close(m0, 1.0, 1e-4):
+        f.append(f"planetary: int zeta rho^2 = {m0}")
+    # PSE second moment: int_0^inf eta(rho) rho^4 drho = 3
+    for k, top in (("winckelmans", 400.0), ("gaussian", 14.0)):
+        m2 = _quad(lambda r: o.vf(k, "eta3", r) * r ** 4, 0.0, top, 40001)
+        tail = 52.5 * (top ** -4) / 4 if k == "winckelmans" else 0.0
+        if not _close(m2 + tail, 3.0, 2e-4):
+            f.append(f"{k}: int eta rho^4 = {m2 + tail}, expected 3")
+    return f
+
+
+def g_cumulative_zeta(o):
+    f = []
+    for k in ("winckelmans", "gaussian", "planetary"):
+        for rho in (0.5, 0.9, 1.5, 3.0):
+            if k == "planetary" and rho >= 1.0:
+                target = 1.0
+            else:
+                target = _quad(lambda r: o.vf(k, "zeta3", r) * r * r, 0.0, rho, 2001)
+            g = o.vf(k, "g3", rho)
+            if not _close(g, target, 2e-5, 2e-7):
+                f.append(f"{k}: g({rho}) = {g} but int_0^rho zeta t^2 = {target}")
+    return f
+
+
+# ---------------------------------------------------------------------------
+# D. 2D kernels
+# ---------------------------------------------------------------------------
+def kernels_2d(o):
+    f = []
+    for rho in (0.05, 0.3, 1.0, 2.5, 7.0):
+        # Gaussian: g_2D = int_0^rho exp(-t^2/2) t dt = 1 - exp(-rho^2/2)
+        tgt = _quad(lambda t: math.exp(-0.5 * t * t) * t, 0.0, rho, 2001)
+        if not _close(o.vf("gaussian", "g2", rho), tgt, 1e-3, 1e-6):
+            f.append(f"gaussian g_2D({rho})")
+        # Winckelmans g_2D: the reference's OpenCL statement of the same kernel
+        # (nbody.cl:554): (rho^2 + 2) rho^2 (rho^2 + 1)^-2
+        q = rho * rho
+        if not _close(o.vf("winckelmans", "g2", rho), (q + 2.0) * q / (q + 1.0) ** 2, 1e-6, 1e-9):
+            f.append(f"winckelmans g_2D({rho}) differs from the OpenCL kernel")
+        # Winckelmans eta_2D: OpenCL statement (nbody.cl:597-599)
+        a = q + 1.0
+        a2 = 1.0 / (a * a)
+        ocl = 24.0 * math.exp(4.0 * a * a2 * a2) * a2 * a2
+        if not _close(o.vf("winckelmans", "eta2", rho), ocl, 1e-5):
+            f.append(f"winckelmans eta_2D({rho}) differs from the OpenCL kernel")
+        if o.vf("planetary", "g2", rho) != (q if rho < 1 else 1.0) and not _close(
+                o.vf("planetary", "g2", rho), q if rho < 1 else 1.0, 1e-6):
+            f.append(f"planetary g_2D({rho})")
+    # 2D PSE second moment: int eta rho^3 drho = 2 (Gaussian); for the
+    # Winckelmans kernel the moment holds for eta * exp(-4/a^3) = 24 / a^4,
+    # the factor exp(4/a^3) being the reference's own (see VortFunc.cpp:129)
+    m = _quad(lambda r: o.vf("gaussian", "eta2", r) * r ** 3, 0.0, 14.0, 20001)
+    if not _close(m, 2.0, 2e-4):
+        f.append(f"gaussian: int eta_2D rho^3 = {m}, expected 2")
+    m = _quad(lambda r: o.vf("winckelmans", "eta2", r) * math.exp(-4.0 / (r * r + 1) ** 3) * r ** 3,
+              0.0, 300.0, 40001)
+    if not _close(m, 2.0, 5e-4):
+        f.append(f"winckelmans: int eta_2D exp(-4/a^3) rho^3 = {m}, expected 2")
+    if not _close(o.vf("winckelmans", "eta2", 0.0), 24.0 * math.exp(4.0), 1e-6):
+        f.append("winckelmans eta_2D(0) != 24 e^4")
+    return f
+
+
+# ---------------------------------------------------------------------------
+# E. reference pair invariants, test/testparticle.h:33-112
+# ---------------------------------------------------------------------------
+def particle_invariants(o):
+    f = []
+    p1 = make_p3d([[0, 0, 0]], [[1, 0, 0]], [1])[0]
+    v0, vx, vy, vz, vzbig = [0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, 100]
+    for k in ("winckelmans", "singular"):
+        s2s = lambda m: o.P3D_S2S_vel(p1, m, k, 1.0)
+        if np.any(s2s(v0) != 0) or np.any(s2s(vx) != 0):
+            f.append(f"{k}: velocity not zero at the particle / on its axis")
+        if np.all(s2s(vy) == 0) or np.all(s2s(vz) == 0):
+            f.append(f"{k}: velocity zero off axis")
+        a, b = s2s(vy), s2s(vz)
+        if not (a[0] == 0 and a[1] == 0 and a[2] > 0 and b[0] == 0 and b[1] < 0 and b[2] == 0):
+            f.append(f"{k}: velocity direction")
+        if not s2s(vz)[1] < s2s(vzbig)[1]:
+            f.append(f"{k}: velocity does not decay")
+    p2 = make_p3d([[0, 0, 0]], [[0, 0, 1]], [1])[0]
+    pxz = make_p3d([[0, 0, 1]], [[1, 0, 0]], [1])[0]
+    pyz = make_p3d([[0, 0, 1]], [[0, 1, 0]], [1])[0]
+    pzz = make_p3d([[0, 0, 1]], [[0, 0, 1]], [1])[0]
+    d = o.P3D_S2S_dvort(p2, pxz, "singular", 1.0)
+    if not (d[0] == 0 and d[1] < 0 and d[2] == 0):
+        f.append("dvort (z on x) sign")
+    d = o.P3D_S2S_dvort(p2, pyz, "singular", 1.0)
+    if not (d[0] > 0 and d[1] == 0 and d[2] == 0):
+        f.append("dvort (z on y) sign")
+    if np.any(o.P3D_S2S_dvort(p2, pzz, "singular", 1.0) != 0):
+        f.append("dvort (z on z) not zero")
+    return f
+
+
+# ---------------------------------------------------------------------------
+# F. singular Biot-Savart, textbook form u = (1/4pi) alpha x (x - x_s) / r^3;
+#    regularised: u_reg = g(rho) u_singular
+# ---------------------------------------------------------------------------
+def biot_savart_pair(o):
+    f = []
+    rng = np.random.default_rng(11)
+    for _ in range(6):
+        xs, a, x = rng.normal(size=3), rng.normal(size=3), rng.normal(size=3)
+        p = make_p3d([xs], [a], [0.3])[0]
+        d = x.astype(np.float32).astype(np.float64) - p["coord"].astype(np.float64)
+        r = np.linalg.norm(d)
+        u = np.cross(p["vorticity"].astype(np.float64), d) / (4 * np.pi * r ** 3)
+        got = o.P3D_S2S_vel(p, x, "singular", 0.5)
+        if np.linalg.norm(got - u) > 1e-5 * np.linalg.norm(u):
+            f.append("singular P3D velocity differs from Biot-Savart")
+        for k in ("winckelmans", "gaussian", "planetary"):
+            g = o.vf64(k, "g3", r / 0.5, 1)
+            got = o.P3D_S2S_vel(p, x, k, 0.5)
+            if np.linalg.norm(got - g * u) > 2e-5 * np.linalg.norm(u):
+                f.append(f"{k} P3D velocity != g(rho) x singular")
+    return f
+
+
+# ---------------------------------------------------------------------------
+# G. transpose-scheme stretching: dvort_pair = grad(alpha_t . u_s)(x_t)
+#    (libcvtx.dox:413-416), checked by central differences of the velocity
+# ---------------------------------------------------------------------------
+def dvort_transpose_fd(o):
+    f = []
+    rng = np.random.default_rng(12)
+    h = 1e-3
+    for _ in range(4):
+        s = make_p3d([rng.uniform(-1, 1, 3)], [rng.normal(size=3)], [0.1])[0]
+        x = rng.uniform(-1, 1, 3)
+        a = rng.normal(size=3)
+        t = make_p3d([x], [a], [0.1])[0]
+        xt = t["coord"].astype(np.float64)
+        at = t["vorticity"].astype(np.float64)
+        for k in REGS:
+            J = np.zeros((3, 3))
+            for j in range(3):
+                e = np.zeros(3)
+                e[j] = h
+                J[:, j] = (o.P3D_S2S_vel(s, xt + e, k, 0.7).astype(np.float64)
+                           - o.P3D_S2S_vel(s, xt - e, k, 0.7)) / (2 * h)
+            want = J.T @ at
+            got = o.P3D_S2S_dvort(s, t, k, 0.7)
+            if np.linalg.norm(got - want) > 2e-3 * np.linalg.norm(want) + 1e-7:
+                f.append(f"{k}: dvort != J^T alpha_t ({got} vs {want})")
+    return f
+
+
+# ---------------------------------------------------------------------------
+# H. PSE: the pair term is antisymmetric, so over sources = targets the
+#    exchange conserves total vorticity; uniform strength gives zero
+# ---------------------------------------------------------------------------
+def pse_conservation(o):
+    f = []
+    p = W.random_p3d(200, 5, box=1.0, vort=1.0, vol=0.01)
+    for k in ("winckelmans", "gaussian"):
+        out = o.P3D_M2M_visc_dvort(p, p, k, 0.3, 0.1).astype(np.float64)
+        tot = np.abs(out.sum(axis=0)).max()
+        scale = np.abs(out).sum(axis=0).max()
+        if not tot <= 1e-5 * scale:
+            f.append(f"{k}: 3D PSE not conservative ({tot} vs {scale})")
+    q = W.random_p2d(200, 6, box=1.0, vort=1.0, area=0.01)
+    for k in ("winckelmans", "gaussian"):
+        out = o.P2D_M2M_visc_dvort(q, q, k, 0.3, 0.1).astype(np.float64)
+        if not abs(out.sum()) <= 1e-5 * np.abs(out).sum():
+            f.append(f"{k}: 2D PSE not conservative")
+    # uniform alpha / volume -> exactly zero
+    u = make_p3d(np.random.default_rng(3).uniform(0, 1, (50, 3)), np.ones((50, 3)), np.full(50, 0.5))
+    for k in ("winckelmans", "gaussian"):
+        if np.abs(o.P3D_M2M_visc_dvort(u, u, k, 0.3, 0.1)).max() > 0:
+            f.append(f"{k}: uniform strengths give non-zero PSE exchange")
+    return f
+
+
+# ---------------------------------------------------------------------------
+# I. straight filament velocity = Gauss-Legendre quadrature of
+#    u(x) = (Gamma/4pi) int dl x (x - l) / |x - l|^3
+# ---------------------------------------------------------------------------
+def _segment_quad(a, b, G, x, n=400):
+    t, w = np.polynomial.legendre.leggauss(n)
+    t = 0.5 * (t + 1.0)
+    w = 0.5 * w
+    dl = b - a
+    pts = a[None, :] + t[:, None] * dl[None, :]
+    r = x[None, :] - pts
+    rn = np.linalg.norm(r, axis=1)
+    integrand = np.cross(np.broadcast_to(dl, r.shape), r) / rn[:, None] ** 3
+    return G / (4 * np.pi) * (w[:, None] * integrand).sum(axis=0)
+
+
+def f3d_vel_quadrature(o):
+    f = []
+    rng = np.random.default_rng(13)
+    for _ in range(6):
+        a, b, x = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3) * 1.5
+        fil = make_f3d([a], [b], [1.7])[0]
+        A = fil["start"].astype(np.float64)
+        B = fil["end"].astype(np.float64)
+        xx = np.asarray(x, np.float32).astype(np.float64)
+        want = _segment_quad(A, B, float(fil["strength"]), xx)
+        got = o.F3D_S2S_vel(fil, xx)
+        if np.linalg.norm(got - want) > 1e-4 * np.linalg.norm(want):
+            f.append(f"F3D velocity != quadrature ({got} vs {want})")
+    # a long segment approaches the infinite line Gamma/(2 pi d)
+    fil = make_f3d([[0, 0, -1e3]], [[0, 0, 1e3]], [2.0])[0]
+    v = o.F3D_S2S_vel(fil, [0.5, 0, 0])
+    if not _close(float(np.linalg.norm(v)), 2.0 / (2 * np.pi * 0.5), 1e-5):
+        f.append("long segment != Gamma/(2 pi d)")
+    # square loop of side L at its centre: 2 sqrt(2) Gamma / (pi L)
+    L = 1.0
+    c = [[0, 0, 0], [L, 0, 0], [L, L, 0], [0, L, 0]]
+    loop = make_f3d([c[0], c[1], c[2], c[3]], [c[1], c[2], c[3], c[0]], [1.0] * 4)
+    v = sum(o.F3D_S2S_vel(loop[i], [0.5, 0.5, 0]).astype(np.float64) for i in range(4))
+    if not _close(float(np.linalg.norm(v)), 2 * math.sqrt(2) / (math.pi * L), 1e-5):
+        f.append("square loop centre velocity")
+    # zero on the extension of the axis (reference F3D.cpp:52-53)
+    if np.any(o.F3D_S2S_vel(fil, [0, 0, 2e3]) != 0):
+        f.append("not zero on the filament axis")
+    return f
+
+
+# ---------------------------------------------------------------------------
+# J. F3D stretching (reference F3D.cpp:56-85).  The formula is reference
+#    defined (no textbook form); pinned by its structure: the map
+#    alpha -> dvort is B I + [A]x with A parallel to the filament and
+#    |A| = |u| / d_perp, u the (quadrature-pinned) filament velocity, and by
+#    B = 0 on the perpendicular bisector
+# ---------------------------------------------------------------------------
+def f3d_dvort_structure(o):
+    f = []
+    rng = np.random.default_rng(14)
+    for _ in range(5):
+        a, b, x = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+        fil = make_f3d([a], [b], [1.3])[0]
+        cols = [o.F3D_S2S_dvort(fil, make_p3d([x], [e], [0.1])[0]).astype(np.float64)
+                for e in np.eye(3)]
+        M = np.array(cols).T
+        Bd = np.trace(M) / 3.0
+        S = 0.5 * (M + M.T)
+        if np.abs(S - Bd * np.eye(3)).max() > 1e-5 * np.abs(M).max():
+            f.append("F3D dvort symmetric part is not isotropic")
+        K = 0.5 * (M - M.T)  # K v = A x v
+        A = np.array([K[2, 1], K[0, 2], K[1, 0]])
+        r0 = fil["end"].astype(np.float64) - fil["start"].astype(np.float64)
+        xx = np.asarray(x, np.float32).astype(np.float64)
+        r1 = xx - fil["start"].astype(np.float64)
+        dperp = np.linalg.norm(np.cross(r0, r1)) / np.linalg.norm(r0)
+        u = o.F3D_S2S_vel(fil, xx).astype(np.float64)
+        # u = (r0 x r1) Gamma t / (4 pi |c|^2) and A = -r0 Gamma t / (4 pi |c|^2),
+        # |c| = |r0| dperp, so A = -(u . n) r0_hat / dperp with n = (r0 x r1)/|c|
+        n = np.cross(r0, r1)
+        n /= np.linalg.norm(n)
+        want = -(u @ n) / dperp * r0 / np.linalg.norm(r0)
+        if np.linalg.norm(A - want) > 1e-4 * np.linalg.norm(want):
+            f.append(f"F3D dvort rotation vector {A} != {want}")
+    # perpendicular bisector: |r1| = |r2| -> B = 0
+    fil = make_f3d([[-1, 0, 0]], [[1, 0, 0]], [1.0])[0]
+    M = np.array([o.F3D_S2S_dvort(fil, make_p3d([[0, 0.5, 0.25]], [e], [0.1])[0]) for e in np.eye(3)]).T
+    if abs(np.trace(M)) > 1e-6 * np.abs(M).max():
+        f.append("F3D dvort isotropic part not zero on the bisector")
+    return f
+
+
+# ---------------------------------------------------------------------------
+# K. 2D point vortex: |u| = Gamma g_2D / (2 pi r), perpendicular to d, with
+#    the reference's orientation u = Gamma/(2 pi r^2) (d_y, -d_x)
+#    (reference P2D.cpp:65-66)
+# ---------------------------------------------------------------------------
+def p2d_point_vortex(o):
+    f = []
+    rng = np.random.default_rng(15)
+    for _ in range(6):
+        xs, x = rng.normal(size=2), rng.normal(size=2)
+        p = make_p2d([xs], [1.9], [0.1])[0]
+        d = np.asarray(x, np.float32).astype(np.float64) - p["coord"].astype(np.float64)
+        r = np.linalg.norm(d)
+        got = o.P2D_S2S_vel(p, x, "singular", 0.3).astype(np.float64)
+        if abs(got @ d) > 1e-5 * np.linalg.norm(got) * r:
+            f.append("2D velocity not perpendicular to separation")
+        if not _close(np.linalg.norm(got), 1.9 / (2 * np.pi * r), 1e-5):
+            f.append("2D point vortex magnitude")
+        if not (got[0] * d[1] >= 0 and got[1] * d[0] <= 0):
+            f.append("2D orientation")
+        for k in ("winckelmans", "gaussian", "planetary"):
+            g = o.vf64(k, "g2", r / 0.3, 1)
+            gk = o.P2D_S2S_vel(p, x, k, 0.3).astype(np.float64)
+            if np.linalg.norm(gk - g * got) > 2e-5 * np.linalg.norm(got):
+                f.append(f"{k}: 2D velocity != g_2D x point vortex")
+    return f
+
+
+# ---------------------------------------------------------------------------
+# L. discrete singular vortex ring of radius R, circulation Gamma: every
+#    particle is at distance R from the centre, so the velocity there is
+#    exactly Gamma/(2R) along the axis for any number of particles
+# ---------------------------------------------------------------------------
+def ring_centre(o):
+    f = []
+    n, R, G = 64, 0.8, 1.3
+    th = 2 * np.pi * (np.arange(n) + 0.25) / n
+    pos = np.stack([R * np.cos(th), R * np.sin(th), np.zeros(n)], 1)
+    alpha = G * 2 * np.pi * R / n * np.stack([-np.sin(th), np.cos(th), np.zeros(n)], 1)
+    p = make_p3d(pos, alpha, np.full(n, 1e-3))
+    v = o.P3D_M2M_vel(p, [[0.0, 0.0, 0.0]], "singular", 0.1)[0].astype(np.float64)
+    if not (_close(v[2], G / (2 * R), 1e-5) and abs(v[0]) < 1e-6 and abs(v[1]) < 1e-6):
+        f.append(f"ring centre velocity {v} != (0, 0, {G / (2 * R)})")
+    return f
+
+
+# ---------------------------------------------------------------------------
+# M. the FP32 restatement agrees with the separately written FP64 oracle on
+#    small many-to-many problems (different code, G/H formulation)
+# ---------------------------------------------------------------------------
+def refcpu_vs_fp64(o):
+    f = []
+    p = W.random_p3d(160, 21, box=1.0, vort=1.0, vol=0.01)
+    t = W.random_p3d(96, 22, box=1.0, vort=1.0, vol=0.01)
+    for k in REGS:
+        v64, d64, s64 = o.f64_P3D(p, t, k, 0.3, nu=0.1, gauss_mode=0, visc=k in ("winckelmans", "gaussian"))
+        v32 = o.P3D_M2M_vel(p, t["coord"], k, 0.3)
+        d32 = o.P3D_M2M_dvort(p, t, k, 0.3)
+        tol = 5e-5 if k == "gaussian" else 5e-6
+        if orc.rel_l2(v32, v64) > 5e-6:
+            f.append(f"{k}: P3D vel FP32 vs FP64 {orc.rel_l2(v32, v64)}")
+        if orc.rel_l2(d32, d64) > tol:
+            f.append(f"{k}: P3D dvort FP32 vs FP64 {orc.rel_l2(d32, d64)}")
+        if s64 is not None:
+            s32 = o.P3D_M2M_visc_dvort(p, t, k, 0.3, 0.1)
+            if orc.rel_l2(s32, s64) > 5e-6:
+                f.append(f"{k}: P3D visc FP32 vs FP64 {orc.rel_l2(s32, s64)}")
+    fl = W.random_f3d(120, 23, box=1.0, strength=1.0)
+    v64, d64 = o.f64_F3D(fl, t)
+    if orc.rel_l2(o.F3D_M2M_vel(fl, t["coord"]), v64) > 5e-6:
+        f.append("F3D vel FP32 vs FP64")
+    if orc.rel_l2(o.F3D_M2M_dvort(fl, t), d64) > 5e-6:
+        f.append("F3D dvort FP32 vs FP64")
+    q = W.random_p2d(160, 24, box=1.0, vort=1.0, area=0.01)
+    r = W.random_p2d(96, 25, box=1.0, vort=1.0, area=0.01)
+    for k in REGS:
+        visc = k in ("winckelmans", "gaussian")
+        v64, e64 = o.f64_P2D(q, r, k, 0.3, nu=0.1, gauss_mode=0, visc=visc)
+        if orc.rel_l2(o.P2D_M2M_vel(q, r["coord"], k, 0.3), v64) > 5e-6:
+            f.append(f"{k}: P2D vel FP32 vs FP64")
+        if visc and orc.rel_l2(o.P2D_M2M_visc_dvort(q, r, k, 0.3, 0.1), e64) > 5e-6:
+            f.append(f"{k}: P2D visc FP32 vs FP64")
+    return f
+
+
+ALL_CHECKS = (kat_vortfunc, moments_3d, g_cumulative_zeta, kernels_2d, particle_invariants,
+              biot_savart_pair, dvort_transpose_fd, pse_conservation, f3d_vel_quadrature,
+              f3d_dvort_structure, p2d_point_vortex, ring_centre, refcpu_vs_fp64)

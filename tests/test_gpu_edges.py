@@ -1,0 +1,148 @@
+"""Edge cases of the GPU path: empty and tiny inputs, sizes that are not
+multiples of the 256-source tile, aliasing of sources and targets,
+coincident particles, negative regularisation radius, every launch shape,
+plain vs compensated accumulation, and the device-resident API against the
+host-array API.  Tolerance: relative L2 <= 1e-5 against the oracle's FP32
+restatement of the reference (written per assertion)."""
+import numpy as np
+import pytest
+
+import cvortex_amd as cv
+from cvortex_amd import device as D
+from cvortex_amd import workloads as W
+from gpuutil import assert_parity, case, have_gpu
+import oracle
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not have_gpu():
+        pytest.fail("no HIP device: GPU tests need one")
+    cv.set_accumulation(cv.ACC_COMP)
+
+
+def test_empty_sources_give_zero():
+    case("empty sources")
+    p = W.random_p3d(0, 1)
+    t = W.random_p3d(300, 2)
+    assert np.all(cv.P3D_M2M_vel(p, t["coord"], "winckelmans", 0.3) == 0)
+    assert np.all(cv.P3D_M2M_dvort(p, t, "winckelmans", 0.3) == 0)
+    f = W.random_f3d(0, 3)
+    assert np.all(cv.F3D_M2M_vel(f, t["coord"]) == 0)
+    q = W.random_p2d(0, 4)
+    assert np.all(cv.P2D_M2M_vel(q, np.ones((5, 2), np.float32), "gaussian", 0.3) == 0)
+
+
+def test_empty_targets_are_a_no_op():
+    case("empty targets")
+    p = W.random_p3d(300, 1)
+    assert cv.P3D_M2M_vel(p, np.zeros((0, 3), np.float32), "winckelmans", 0.3).shape == (0, 3)
+    assert cv.P3D_M2M_dvort(p, W.random_p3d(0, 2), "gaussian", 0.3).shape == (0, 3)
+
+
+@pytest.mark.parametrize("ns,nt", [(1, 1), (1, 700), (255, 3), (256, 256), (257, 1),
+                                   (513, 1031), (4097, 65), (70000, 129)])
+def test_ragged_sizes(orc, ns, nt):
+    case(f"ragged ns={ns} nt={nt}")
+    p = W.random_p3d(ns, 10 + ns, box=1.0, vort=1.0)
+    t = W.random_p3d(nt, 20 + nt, box=1.0, vort=1.0)
+    for k in ("singular", "winckelmans", "gaussian"):
+        assert_parity(f"vel {k}", cv.P3D_M2M_vel(p, t["coord"], k, 0.2),
+                      orc.P3D_M2M_vel(p, t["coord"], k, 0.2), TOL)
+        assert_parity(f"dvort {k}", cv.P3D_M2M_dvort(p, t, k, 0.2), orc.P3D_M2M_dvort(p, t, k, 0.2), TOL)
+
+
+def test_coincident_particles_are_excluded(orc):
+    # duplicated coordinates with different strengths: the reference skips
+    # every pair with equal coordinates (src/P3D.cpp:58,94,127)
+    case("coincident particles")
+    p = W.random_p3d(600, 31, box=1.0, vort=1.0)
+    p["coord"][300:] = p["coord"][:300]
+    for k in ("singular", "winckelmans", "planetary", "gaussian"):
+        v = cv.P3D_M2M_vel(p, p["coord"], k, 0.2)
+        d = cv.P3D_M2M_dvort(p, p, k, 0.2)
+        assert np.all(np.isfinite(v)) and np.all(np.isfinite(d))
+        assert_parity(f"vel {k}", v, orc.P3D_M2M_vel(p, p["coord"], k, 0.2), TOL)
+        assert_parity(f"dvort {k}", d, orc.P3D_M2M_dvort(p, p, k, 0.2), TOL)
+        if k in ("winckelmans", "gaussian"):
+            s = cv.P3D_M2M_visc_dvort(p, p, k, 0.2, 0.05)
+            assert_parity(f"visc {k}", s, orc.P3D_M2M_visc_dvort(p, p, k, 0.2, 0.05), TOL)
+
+
+def test_negative_radius(orc):
+    # CPU semantics: vel uses |sigma|, dvort the signed sigma^3 (P3D.cpp:82,102)
+    case("negative sigma")
+    p = W.random_p3d(500, 41, box=1.0, vort=1.0)
+    for k in ("winckelmans", "gaussian", "singular"):
+        assert_parity(f"vel {k}", cv.P3D_M2M_vel(p, p["coord"], k, -0.2),
+                      orc.P3D_M2M_vel(p, p["coord"], k, -0.2), TOL)
+        assert_parity(f"dvort {k}", cv.P3D_M2M_dvort(p, p, k, -0.2),
+                      orc.P3D_M2M_dvort(p, p, k, -0.2), TOL)
+
+
+@pytest.mark.parametrize("T", [1, 2, 4])
+@pytest.mark.parametrize("splits", [1, 3, 16])
+def test_every_launch_shape(orc, T, splits):
+    case(f"shape T={T} splits={splits}")
+    p = W.random_p3d(3000, 51, box=1.0, vort=1.0)
+    dev = D.P3DDevice(p)
+    for k in ("winckelmans", "gaussian"):
+        outs = dev.outputs(True, True, True)
+        D.p3d_eval(dev.problem(outs, k, 0.1, 0.05), shape=(T, splits))
+        v, d, s = (o.download() for o in outs)
+        assert_parity(f"vel {k}", v, orc.P3D_M2M_vel(p, p["coord"], k, 0.1), TOL)
+        assert_parity(f"dvort {k}", d, orc.P3D_M2M_dvort(p, p, k, 0.1), TOL)
+        assert_parity(f"visc {k}", s, orc.P3D_M2M_visc_dvort(p, p, k, 0.1, 0.05), TOL)
+
+
+def test_plain_accumulation(orc):
+    case("plain accumulation")
+    p = W.random_p3d(5000, 61, box=1.0, vort=1.0)
+    prev = cv.set_accumulation(cv.ACC_PLAIN)
+    try:
+        for k in ("winckelmans", "gaussian"):
+            assert_parity(f"vel {k}", cv.P3D_M2M_vel(p, p["coord"], k, 0.1),
+                          orc.P3D_M2M_vel(p, p["coord"], k, 0.1), TOL)
+    finally:
+        cv.set_accumulation(prev)
+
+
+def test_device_api_matches_host_api():
+    case("device API == host API")
+    p = W.random_p3d(2000, 71, box=1.0, vort=1.0)
+    dev = D.P3DDevice(p)
+    outs = dev.outputs(True, True, False)
+    D.p3d_eval(dev.problem(outs, "winckelmans", 0.1))
+    v, d = outs[0].download(), outs[1].download()
+    hv, hd, _ = cv.P3D_M2M_fused(p, p, "winckelmans", 0.1, 0.0, vel=True, dvort=True)
+    assert np.array_equal(v, hv) and np.array_equal(d, hd)
+
+
+def test_f3d_p2d_device_shapes(orc):
+    case("F3D/P2D device shapes")
+    f = W.random_f3d(700, 81, box=1.0, strength=1.0)
+    t = W.random_p3d(900, 82, box=1.0, vort=1.0)
+    fa, fb = D.f3d_soa(f)
+    tpv, ta = D.p3d_soa(t)
+    dfa, dfb = D.DeviceArray.from_host(fa), D.DeviceArray.from_host(fb)
+    dtp, dta = D.DeviceArray.from_host(tpv), D.DeviceArray.from_host(ta)
+    for shape in ((1, 1), (2, 2), (4, 5)):
+        v = D.DeviceArray((900, 3))
+        dv = D.DeviceArray((900, 3))
+        D.f3d_eval(D.F3DProblem(dfa.addr, dfb.addr, 700, dtp.addr, dta.addr, 900, v.addr, dv.addr, 1),
+                   shape=shape)
+        assert_parity(f"F3D vel {shape}", v.download(), orc.F3D_M2M_vel(f, t["coord"]), TOL)
+        assert_parity(f"F3D dvort {shape}", dv.download(), orc.F3D_M2M_dvort(f, t), TOL)
+    q = W.random_p2d(800, 83, box=1.0, vort=1.0)
+    dq = D.DeviceArray.from_host(D.p2d_soa(q))
+    for shape in ((1, 1), (2, 3), (4, 2)):
+        v = D.DeviceArray((800, 2))
+        e = D.DeviceArray((800,))
+        D.p2d_eval(D.P2DProblem(dq.addr, 800, dq.addr, 800, v.addr, e.addr, 1, 0.1, 0.05, 1),
+                   shape=shape)
+        assert_parity(f"P2D vel {shape}", v.download(), orc.P2D_M2M_vel(q, q["coord"], "winckelmans", 0.1), TOL)
+        assert_parity(f"P2D visc {shape}", e.download(),
+                      orc.P2D_M2M_visc_dvort(q, q, "winckelmans", 0.1, 0.05), TOL)
