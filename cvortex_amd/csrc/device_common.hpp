@@ -144,6 +144,7 @@ struct RegKernel<REG_GAUSSIAN> {
     float t = q * pinv * (float)GTAB_PER_UNIT;        // rho * 16
     int i = (int)t;
     i = i < (GTAB_N - 1) ? i : (GTAB_N - 1);
+    i = i > 0 ? i : 0;
     float u = t - (float)i;
     float4 cg = tab[2 * i];
     float4 ch = tab[2 * i + 1];

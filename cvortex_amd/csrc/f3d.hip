@@ -2,18 +2,26 @@
 //
 // Filaments stream through LDS in tiles; when a tile is staged each
 // filament is expanded once per workgroup into
-//   { a (start), Gamma/(4pi) }  and  { r0 = b - a, 3 Gamma / (4pi |r0|) },
+//   { a (start), Gamma/(4pi) },  { b (end), 3 Gamma/(4pi |r0|) },  { r0 = b - a },
 // so the per-pair work only involves the target-dependent quantities.
 //
-// Per pair (x the target): r1 = x - a, r2 = r1 - r0 (= x - b), c = r0 x r1
-// (= r1 x r2), t = (r0.r1)/|r1| - (r0.r2)/|r2|.
-//   vel:   v += (Gamma/4pi) t / |c|^2 * c              (reference F3D.cpp:34-54)
-//   dvort: B += 3 Gamma/(4pi|r0|) |c| (1/|r1| - 1/|r2|),
-//          S += (Gamma/4pi) t / |c|^2 * r0
-//          dvort = a_t B - S x a_t                       (reference F3D.cpp:56-85)
-// A pair whose terms are not finite (target on a filament's axis or end
-// point) contributes zero, as the reference's GPU kernels do
-// (nbody.cl:637,718).
+// Per pair (x the target): r1 = x - a, r2 = x - b, c = r1 x r2 (= r0 x r1),
+// P = |r1||r2|, s = |r1| + |r2|, d = r1.r2.  The reference's factors
+// (F3D.cpp:34-85)
+//   t2 = r0.r1/|r1| - r0.r2/|r2|            and   1/|r1| - 1/|r2|
+// cancel in FP32 for targets far from the segment or near its end points,
+// so they are evaluated in the algebraically identical forms
+//   k = (Gamma/4pi) t2 / |c|^2 = (Gamma/4pi) s / (P (P + d))          (d >= 0)
+//                              = (Gamma/4pi) s (P - d) / (P |c|^2)    (d < 0)
+//   1/|r1| - 1/|r2| = -r0.(r1 + r2) / (P s)
+// (using |c|^2 = P^2 - d^2 and |r2|^2 - |r1|^2 = -r0.(r1 + r2)).  Then
+//   vel:   v += k c
+//   dvort: B += 3 Gamma/(4pi|r0|) |c| (1/|r1| - 1/|r2|),  S += k r0,
+//          dvort = a_t B - S x a_t
+// A pair whose terms are not finite (target on a filament's line or an end
+// point) contributes zero, as in the reference's GPU kernels
+// (nbody.cl:637,718); on the line's extension the velocity term is exactly
+// zero there (c = 0) and the stretching term is dropped.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -57,6 +65,7 @@ __global__ __launch_bounds__(WG) void f3d_kernel(F3DKParams p) {
   constexpr int NA = ((MASK & OUT_VEL) ? 3 : 0) + ((MASK & OUT_DVORT) ? 4 : 0);
   constexpr int D = (MASK & OUT_VEL) ? 3 : 0;
   __shared__ float4 s_a[WG];
+  __shared__ float4 s_b[WG];
   __shared__ float4 s_r0[WG];
   const int tid = threadIdx.x;
   const int base = blockIdx.x * (WG * T);
@@ -93,7 +102,8 @@ __global__ __launch_bounds__(WG) void f3d_kernel(F3DKParams p) {
       const float g = a.w * k4pi;
       const float len = __builtin_sqrtf(rx * rx + ry * ry + rz * rz);
       s_a[tid] = make_float4(a.x, a.y, a.z, g);
-      s_r0[tid] = make_float4(rx, ry, rz, 3.f * g / len);
+      s_b[tid] = make_float4(b.x, b.y, b.z, 3.f * g / len);
+      s_r0[tid] = make_float4(rx, ry, rz, 0.f);
     }
     __syncthreads();
     if constexpr (ACC == ACC_COMP) {
@@ -105,42 +115,50 @@ __global__ __launch_bounds__(WG) void f3d_kernel(F3DKParams p) {
 #pragma unroll 2
     for (int j = 0; j < cnt; ++j) {
       const float4 fa = s_a[j];
+      const float4 fb = s_b[j];
       const float4 f0 = s_r0[j];
 #pragma unroll
       for (int k = 0; k < T; ++k) {
         const float r1x = xt[k].x - fa.x, r1y = xt[k].y - fa.y, r1z = xt[k].z - fa.z;
-        const float r2x = r1x - f0.x, r2y = r1y - f0.y, r2z = r1z - f0.z;
-        // c = r0 x r1
-        const float cx = __builtin_fmaf(f0.y, r1z, -f0.z * r1y);
-        const float cy = __builtin_fmaf(f0.z, r1x, -f0.x * r1z);
-        const float cz = __builtin_fmaf(f0.x, r1y, -f0.y * r1x);
+        const float r2x = xt[k].x - fb.x, r2y = xt[k].y - fb.y, r2z = xt[k].z - fb.z;
+        // c = r1 x r2 (= r0 x r1); for targets beside the segment its
+        // products are smaller than those of r0 x r1, so it loses less to
+        // cancellation near the filament
+        const float cx = __builtin_fmaf(r1y, r2z, -r1z * r2y);
+        const float cy = __builtin_fmaf(r1z, r2x, -r1x * r2z);
+        const float cz = __builtin_fmaf(r1x, r2y, -r1y * r2x);
         const float cc = __builtin_fmaf(cx, cx, __builtin_fmaf(cy, cy, cz * cz));
-        const float n1 = __builtin_fmaf(r1x, r1x, __builtin_fmaf(r1y, r1y, r1z * r1z));
-        const float n2 = __builtin_fmaf(r2x, r2x, __builtin_fmaf(r2y, r2y, r2z * r2z));
-        const float i1 = __builtin_amdgcn_rsqf(n1);
-        const float i2 = __builtin_amdgcn_rsqf(n2);
-        const float p1 = __builtin_fmaf(f0.x, r1x, __builtin_fmaf(f0.y, r1y, f0.z * r1z));
-        const float p2 = __builtin_fmaf(f0.x, r2x, __builtin_fmaf(f0.y, r2y, f0.z * r2z));
-        const float t = __builtin_fmaf(p1, i1, -p2 * i2);
+        const float n1s = __builtin_fmaf(r1x, r1x, __builtin_fmaf(r1y, r1y, r1z * r1z));
+        const float n2s = __builtin_fmaf(r2x, r2x, __builtin_fmaf(r2y, r2y, r2z * r2z));
+        const float n1 = n1s * __builtin_amdgcn_rsqf(n1s);
+        const float n2 = n2s * __builtin_amdgcn_rsqf(n2s);
+        const float P = n1 * n2;
+        const float sum = n1 + n2;
+        const float d12 = __builtin_fmaf(r1x, r2x, __builtin_fmaf(r1y, r2y, r1z * r2z));
+        const bool same_side = d12 >= 0.f;
+        const float den = same_side ? P * (P + d12) : P * cc;
+        const float num = same_side ? sum : sum * (P - d12);
+        float kk = fa.w * num * __builtin_amdgcn_rcpf(den);
         if constexpr ((MASK & OUT_DVORT) != 0) {
-          const float rc = __builtin_amdgcn_rsqf(cc);
-          const float icc = rc * rc;
-          float kk = fa.w * t * icc;
-          float B = f0.w * (cc * rc) * (i1 - i2);
-          const bool ok = (__builtin_fabsf(kk) < __builtin_inff()) && (__builtin_fabsf(B) < __builtin_inff());
-          kk = ok ? kk : 0.f;
+          const float rcc = __builtin_amdgcn_rsqf(cc);
+          const float cn = cc * rcc;  // |c|
+          const float p12 = __builtin_fmaf(f0.x, r1x + r2x, __builtin_fmaf(f0.y, r1y + r2y, f0.z * (r1z + r2z)));
+          float B = -fb.w * cn * p12 * __builtin_amdgcn_rcpf(P * sum);
+          const bool ok = (cc > 0.f) && (__builtin_fabsf(kk) < __builtin_inff()) &&
+                          (__builtin_fabsf(B) < __builtin_inff());
           B = ok ? B : 0.f;
+          const float kd = ok ? kk : 0.f;
           if constexpr ((MASK & OUT_VEL) != 0) {
+            kk = (__builtin_fabsf(kk) < __builtin_inff()) ? kk : 0.f;
             acc[k][0] = __builtin_fmaf(kk, cx, acc[k][0]);
             acc[k][1] = __builtin_fmaf(kk, cy, acc[k][1]);
             acc[k][2] = __builtin_fmaf(kk, cz, acc[k][2]);
           }
           acc[k][D + 0] += B;
-          acc[k][D + 1] = __builtin_fmaf(kk, f0.x, acc[k][D + 1]);
-          acc[k][D + 2] = __builtin_fmaf(kk, f0.y, acc[k][D + 2]);
-          acc[k][D + 3] = __builtin_fmaf(kk, f0.z, acc[k][D + 3]);
+          acc[k][D + 1] = __builtin_fmaf(kd, f0.x, acc[k][D + 1]);
+          acc[k][D + 2] = __builtin_fmaf(kd, f0.y, acc[k][D + 2]);
+          acc[k][D + 3] = __builtin_fmaf(kd, f0.z, acc[k][D + 3]);
         } else {
-          float kk = fa.w * t * __builtin_amdgcn_rcpf(cc);
           kk = (__builtin_fabsf(kk) < __builtin_inff()) ? kk : 0.f;
           acc[k][0] = __builtin_fmaf(kk, cx, acc[k][0]);
           acc[k][1] = __builtin_fmaf(kk, cy, acc[k][1]);

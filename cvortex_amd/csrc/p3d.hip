@@ -26,6 +26,7 @@
 
 namespace cvtx {
 
+// Canonical layout of the per-target sums (FP64 partials and epilogue).
 template <int MASK>
 struct P3DLayout {
   static constexpr int VEL = 0;
@@ -35,6 +36,27 @@ struct P3DLayout {
   static constexpr int EV = E + 3;                          // sum eta V_s
   static constexpr int N = E + ((MASK & OUT_VISC) ? 4 : 0);
   static constexpr bool NEED_TA = (MASK & (OUT_DVORT | OUT_VISC)) != 0;
+};
+
+// Register layout of the accumulators.  FP32 accumulators hold the sums
+// whose terms carry no coherent large component: vel (sum Kv d x a_s) and
+// the gradient term (sum Kh (a_t.c) d).  With ACC_COMP the stretching sum
+// sum Kv a_s and the PSE sums sum eta a_s, sum eta V_s are accumulated per
+// pair in FP64 (full rate on CDNA4): their near-field terms are large and
+// nearly parallel to a_t, and only their small component normal to a_t (or
+// the small difference V_t E - a_t e) survives, so FP32 tile sums would
+// lose several digits there.  With ACC_PLAIN everything is FP32.
+template <int MASK, int ACC>
+struct P3DRegs {
+  static constexpr bool DBL = (ACC == ACC_COMP);
+  static constexpr int F_VEL = 0;
+  static constexpr int F_A = (MASK & OUT_VEL) ? 3 : 0;                        // plain only
+  static constexpr int F_H = F_A + ((!DBL && (MASK & OUT_DVORT)) ? 3 : 0);
+  static constexpr int F_E = F_H + ((MASK & OUT_DVORT) ? 3 : 0);              // plain only
+  static constexpr int NF = F_E + ((!DBL && (MASK & OUT_VISC)) ? 4 : 0);
+  static constexpr int D_A = 0;
+  static constexpr int D_E = (MASK & OUT_DVORT) ? 3 : 0;
+  static constexpr int ND = DBL ? (D_E + ((MASK & OUT_VISC) ? 4 : 0)) : 0;
 };
 
 int p3d_nacc(int mask) {
@@ -56,11 +78,12 @@ struct P3DKParams {
   Scales sc;
 };
 
-template <int REG, int MASK>
-__device__ __forceinline__ void p3d_pair(const float4 sp, const float4 sa, const float3 xt,
-                                         const float3 at, const RegConsts& rc,
-                                         const float4* tab, float* acc) {
-  using L = P3DLayout<MASK>;
+template <int REG, int MASK, int ACC>
+__device__ __forceinline__ void p3d_pair(const float4 sp, const float4 sa, const double2 sxy,
+                                         const double2 szv, const float3 xt, const float3 at,
+                                         const RegConsts& rc, const float4* tab, float* acc,
+                                         double* dacc) {
+  using R = P3DRegs<MASK, ACC>;
   const float dx = xt.x - sp.x;
   const float dy = xt.y - sp.y;
   const float dz = xt.z - sp.z;
@@ -73,25 +96,40 @@ __device__ __forceinline__ void p3d_pair(const float4 sp, const float4 sa, const
     const float cy = __builtin_fmaf(dz, sa.x, -dx * sa.z);
     const float cz = __builtin_fmaf(dx, sa.y, -dy * sa.x);
     if constexpr ((MASK & OUT_VEL) != 0) {
-      acc[L::VEL + 0] = __builtin_fmaf(Kv, cx, acc[L::VEL + 0]);
-      acc[L::VEL + 1] = __builtin_fmaf(Kv, cy, acc[L::VEL + 1]);
-      acc[L::VEL + 2] = __builtin_fmaf(Kv, cz, acc[L::VEL + 2]);
+      acc[R::F_VEL + 0] = __builtin_fmaf(Kv, cx, acc[R::F_VEL + 0]);
+      acc[R::F_VEL + 1] = __builtin_fmaf(Kv, cy, acc[R::F_VEL + 1]);
+      acc[R::F_VEL + 2] = __builtin_fmaf(Kv, cz, acc[R::F_VEL + 2]);
     }
     if constexpr ((MASK & OUT_DVORT) != 0) {
-      acc[L::A + 0] = __builtin_fmaf(Kv, sa.x, acc[L::A + 0]);
-      acc[L::A + 1] = __builtin_fmaf(Kv, sa.y, acc[L::A + 1]);
-      acc[L::A + 2] = __builtin_fmaf(Kv, sa.z, acc[L::A + 2]);
+      if constexpr (R::DBL) {
+        const double k = (double)Kv;
+        dacc[R::D_A + 0] = __builtin_fma(k, sxy.x, dacc[R::D_A + 0]);
+        dacc[R::D_A + 1] = __builtin_fma(k, sxy.y, dacc[R::D_A + 1]);
+        dacc[R::D_A + 2] = __builtin_fma(k, szv.x, dacc[R::D_A + 2]);
+      } else {
+        acc[R::F_A + 0] = __builtin_fmaf(Kv, sa.x, acc[R::F_A + 0]);
+        acc[R::F_A + 1] = __builtin_fmaf(Kv, sa.y, acc[R::F_A + 1]);
+        acc[R::F_A + 2] = __builtin_fmaf(Kv, sa.z, acc[R::F_A + 2]);
+      }
       const float t = Kh * __builtin_fmaf(at.x, cx, __builtin_fmaf(at.y, cy, at.z * cz));
-      acc[L::H + 0] = __builtin_fmaf(t, dx, acc[L::H + 0]);
-      acc[L::H + 1] = __builtin_fmaf(t, dy, acc[L::H + 1]);
-      acc[L::H + 2] = __builtin_fmaf(t, dz, acc[L::H + 2]);
+      acc[R::F_H + 0] = __builtin_fmaf(t, dx, acc[R::F_H + 0]);
+      acc[R::F_H + 1] = __builtin_fmaf(t, dy, acc[R::F_H + 1]);
+      acc[R::F_H + 2] = __builtin_fmaf(t, dz, acc[R::F_H + 2]);
     }
   }
   if constexpr ((MASK & OUT_VISC) != 0) {
-    acc[L::E + 0] = __builtin_fmaf(eta, sa.x, acc[L::E + 0]);
-    acc[L::E + 1] = __builtin_fmaf(eta, sa.y, acc[L::E + 1]);
-    acc[L::E + 2] = __builtin_fmaf(eta, sa.z, acc[L::E + 2]);
-    acc[L::EV] = __builtin_fmaf(eta, sp.w, acc[L::EV]);
+    if constexpr (R::DBL) {
+      const double e = (double)eta;
+      dacc[R::D_E + 0] = __builtin_fma(e, sxy.x, dacc[R::D_E + 0]);
+      dacc[R::D_E + 1] = __builtin_fma(e, sxy.y, dacc[R::D_E + 1]);
+      dacc[R::D_E + 2] = __builtin_fma(e, szv.x, dacc[R::D_E + 2]);
+      dacc[R::D_E + 3] = __builtin_fma(e, szv.y, dacc[R::D_E + 3]);
+    } else {
+      acc[R::F_E + 0] = __builtin_fmaf(eta, sa.x, acc[R::F_E + 0]);
+      acc[R::F_E + 1] = __builtin_fmaf(eta, sa.y, acc[R::F_E + 1]);
+      acc[R::F_E + 2] = __builtin_fmaf(eta, sa.z, acc[R::F_E + 2]);
+      acc[R::F_E + 3] = __builtin_fmaf(eta, sp.w, acc[R::F_E + 3]);
+    }
   }
 }
 
@@ -127,9 +165,13 @@ __device__ __forceinline__ void p3d_finish(const double* a, float3 at, float vt,
 template <int REG, int MASK, int ACC, int T, int WG>
 __global__ __launch_bounds__(WG) void p3d_kernel(P3DKParams p) {
   using L = P3DLayout<MASK>;
-  constexpr int NA = L::N;
+  using R = P3DRegs<MASK, ACC>;
+  constexpr int NF = R::NF;
+  constexpr int ND = R::ND > 0 ? R::ND : 1;
   __shared__ float4 s_pv[WG];
   __shared__ float4 s_a[WG];
+  __shared__ double2 s_xy[R::ND > 0 ? WG : 1];
+  __shared__ double2 s_zv[R::ND > 0 ? WG : 1];
   __shared__ float4 s_tab[REG == REG_GAUSSIAN ? 2 * GTAB_N : 1];
 
   const int tid = threadIdx.x;
@@ -155,13 +197,17 @@ __global__ __launch_bounds__(WG) void p3d_kernel(P3DKParams p) {
     }
   }
 
-  float acc[T][NA];
-  float lo[T][NA];
-  float hi[T][NA];
+  float acc[T][NF];
+  float lo[T][NF];
+  float hi[T][NF];
+  double dacc[T][ND];
 #pragma unroll
-  for (int k = 0; k < T; ++k)
+  for (int k = 0; k < T; ++k) {
 #pragma unroll
-    for (int a = 0; a < NA; ++a) { acc[k][a] = 0.f; lo[k][a] = 0.f; hi[k][a] = 0.f; }
+    for (int a = 0; a < NF; ++a) { acc[k][a] = 0.f; lo[k][a] = 0.f; hi[k][a] = 0.f; }
+#pragma unroll
+    for (int a = 0; a < ND; ++a) dacc[k][a] = 0.0;
+  }
 
   const int s0 = blockIdx.y * p.split_len;
   const int s1 = min(s0 + p.split_len, p.n_src);
@@ -169,28 +215,40 @@ __global__ __launch_bounds__(WG) void p3d_kernel(P3DKParams p) {
     const int cnt = min(WG, s1 - ts);
     __syncthreads();
     if (tid < cnt) {
-      s_pv[tid] = p.src_pv[ts + tid];
-      s_a[tid] = p.src_a[ts + tid];
+      const float4 q = p.src_pv[ts + tid];
+      const float4 w = p.src_a[ts + tid];
+      s_pv[tid] = q;
+      s_a[tid] = w;
+      if constexpr (R::ND > 0) {
+        s_xy[tid] = make_double2((double)w.x, (double)w.y);
+        s_zv[tid] = make_double2((double)w.z, (double)q.w);
+      }
     }
     __syncthreads();
     if constexpr (ACC == ACC_COMP) {
 #pragma unroll
       for (int k = 0; k < T; ++k)
 #pragma unroll
-        for (int a = 0; a < NA; ++a) acc[k][a] = 0.f;
+        for (int a = 0; a < NF; ++a) acc[k][a] = 0.f;
     }
 #pragma unroll 2
     for (int j = 0; j < cnt; ++j) {
       const float4 sp = s_pv[j];
       const float4 sa = s_a[j];
+      double2 sxy = make_double2(0.0, 0.0), szv = make_double2(0.0, 0.0);
+      if constexpr (R::ND > 0) {
+        sxy = s_xy[j];
+        szv = s_zv[j];
+      }
 #pragma unroll
-      for (int k = 0; k < T; ++k) p3d_pair<REG, MASK>(sp, sa, xt[k], at[k], p.rc, s_tab, acc[k]);
+      for (int k = 0; k < T; ++k)
+        p3d_pair<REG, MASK, ACC>(sp, sa, sxy, szv, xt[k], at[k], p.rc, s_tab, acc[k], dacc[k]);
     }
     if constexpr (ACC == ACC_COMP) {
 #pragma unroll
       for (int k = 0; k < T; ++k)
 #pragma unroll
-        for (int a = 0; a < NA; ++a) two_sum_add(hi[k][a], lo[k][a], acc[k][a]);
+        for (int a = 0; a < NF; ++a) two_sum_add(hi[k][a], lo[k][a], acc[k][a]);
     }
   }
 
@@ -198,16 +256,32 @@ __global__ __launch_bounds__(WG) void p3d_kernel(P3DKParams p) {
   for (int k = 0; k < T; ++k) {
     const int i = base + k * WG + tid;
     if (i >= p.n_tgt) continue;
-    double d[NA];
+    // gather the canonical layout
+    double d[L::N];
+    auto fval = [&](int a) -> double {
+      return (ACC == ACC_COMP) ? (double)hi[k][a] + (double)lo[k][a] : (double)acc[k][a];
+    };
+    if constexpr ((MASK & OUT_VEL) != 0) {
 #pragma unroll
-    for (int a = 0; a < NA; ++a)
-      d[a] = (ACC == ACC_COMP) ? (double)hi[k][a] + (double)lo[k][a] : (double)acc[k][a];
+      for (int c = 0; c < 3; ++c) d[L::VEL + c] = fval(R::F_VEL + c);
+    }
+    if constexpr ((MASK & OUT_DVORT) != 0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        d[L::A + c] = R::DBL ? dacc[k][R::D_A + c] : fval(R::F_A + c);
+        d[L::H + c] = fval(R::F_H + c);
+      }
+    }
+    if constexpr ((MASK & OUT_VISC) != 0) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) d[L::E + c] = R::DBL ? dacc[k][R::D_E + c] : fval(R::F_E + c);
+    }
     if (p.n_split == 1) {
       p3d_finish<MASK>(d, at[k], vt[k], p.sc, p.out_vel, p.out_dvort, p.out_visc, i);
     } else {
 #pragma unroll
-      for (int a = 0; a < NA; ++a)
-        p.partial[((size_t)blockIdx.y * NA + a) * p.n_tgt + i] = d[a];
+      for (int a = 0; a < L::N; ++a)
+        p.partial[((size_t)blockIdx.y * L::N + a) * p.n_tgt + i] = d[a];
     }
   }
 }

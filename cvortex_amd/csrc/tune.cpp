@@ -204,4 +204,10 @@ int cvtx_hip_tune_load(const char* path) {
 
 void cvtx_hip_tune_clear(void) { cache().clear(); }
 
+int cvtx_hip_gaussian_table(float* out, int capacity) {
+  if (out == nullptr || capacity < 8 * GTAB_N) return -1;
+  build_gaussian_table(out);
+  return 8 * GTAB_N;
+}
+
 }  // extern "C"

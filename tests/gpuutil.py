@@ -37,3 +37,30 @@ def assert_parity(name, got, ref, tol=1e-5):
     assert np.all(np.isfinite(got)), f"{name}: non-finite GPU output"
     e = report(name, got, ref)
     assert e <= tol, f"{name}: relative L2 {e:.3e} > {tol:g}"
+
+
+def assert_parity_limited(name, got, ref32, exact, tol=1e-5):
+    """Parity where the reference's own FP32 evaluation is the limit.
+
+    Two pair terms of the reference lose accuracy in FP32 by themselves:
+    the Gaussian g(rho), evaluated with the Abramowitz & Stegun erf
+    (src/VortFunc.cpp:155-176), and the filament stretching, whose |r1 x r0|
+    cancels near a filament (src/F3D.cpp:68-73).  The GPU evaluates both
+    more accurately, so its distance to the reference is the reference's
+    own error.  Require: the GPU matches the FP64 oracle to `tol`, and is no
+    farther from the reference than max(tol, 1.5 x the reference's distance
+    from the FP64 oracle).
+    """
+    got = np.asarray(got)
+    assert np.all(np.isfinite(got)), f"{name}: non-finite GPU output"
+    e_ex = oracle.rel_l2(got, exact)
+    e_ref = oracle.rel_l2(got, ref32)
+    d_ref = oracle.rel_l2(ref32, exact)
+    print(f"  {name}: relL2 vs FP64 {e_ex:.2e}, vs ref {e_ref:.2e} (ref vs FP64 {d_ref:.2e})",
+          flush=True)
+    assert e_ex <= tol, f"{name}: relative L2 vs FP64 oracle {e_ex:.3e} > {tol:g}"
+    assert e_ref <= max(tol, 1.5 * d_ref), (
+        f"{name}: relative L2 vs reference {e_ref:.3e} > max({tol:g}, 1.5 x {d_ref:.3e})")
+
+
+assert_parity_gaussian = assert_parity_limited

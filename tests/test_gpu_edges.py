@@ -10,7 +10,7 @@ import pytest
 import cvortex_amd as cv
 from cvortex_amd import device as D
 from cvortex_amd import workloads as W
-from gpuutil import assert_parity, case, have_gpu
+from gpuutil import assert_parity, assert_parity_gaussian, assert_parity_limited, case, have_gpu
 import oracle
 
 pytestmark = pytest.mark.gpu
@@ -22,6 +22,24 @@ def gpu():
     if not have_gpu():
         pytest.fail("no HIP device: GPU tests need one")
     cv.set_accumulation(cv.ACC_COMP)
+
+
+def check(orc, k, p, t, sigma, visc_nu=None, acc_note=""):
+    """GPU vel/dvort(/visc) of sources p on targets t against the oracle."""
+    v, d, s = cv.P3D_M2M_fused(p, t, k, sigma, visc_nu or 0.0, vel=True, dvort=True,
+                               visc=visc_nu is not None)
+    refs = (orc.P3D_M2M_vel(p, t["coord"], k, sigma), orc.P3D_M2M_dvort(p, t, k, sigma),
+            orc.P3D_M2M_visc_dvort(p, t, k, sigma, visc_nu) if visc_nu is not None else None)
+    names = (f"vel {k}{acc_note}", f"dvort {k}{acc_note}", f"visc {k}{acc_note}")
+    if k == "gaussian":
+        ex = orc.f64_P3D(p, t, k, sigma, visc_nu or 0.0, gauss_mode=1, visc=visc_nu is not None)
+        for n_, g, r, e in zip(names, (v, d, s), refs, ex):
+            if g is not None:
+                assert_parity_gaussian(n_, g, r, e, TOL)
+    else:
+        for n_, g, r in zip(names, (v, d, s), refs):
+            if g is not None:
+                assert_parity(n_, g, r, TOL)
 
 
 def test_empty_sources_give_zero():
@@ -50,9 +68,7 @@ def test_ragged_sizes(orc, ns, nt):
     p = W.random_p3d(ns, 10 + ns, box=1.0, vort=1.0)
     t = W.random_p3d(nt, 20 + nt, box=1.0, vort=1.0)
     for k in ("singular", "winckelmans", "gaussian"):
-        assert_parity(f"vel {k}", cv.P3D_M2M_vel(p, t["coord"], k, 0.2),
-                      orc.P3D_M2M_vel(p, t["coord"], k, 0.2), TOL)
-        assert_parity(f"dvort {k}", cv.P3D_M2M_dvort(p, t, k, 0.2), orc.P3D_M2M_dvort(p, t, k, 0.2), TOL)
+        check(orc, k, p, t, 0.2)
 
 
 def test_coincident_particles_are_excluded(orc):
@@ -62,14 +78,7 @@ def test_coincident_particles_are_excluded(orc):
     p = W.random_p3d(600, 31, box=1.0, vort=1.0)
     p["coord"][300:] = p["coord"][:300]
     for k in ("singular", "winckelmans", "planetary", "gaussian"):
-        v = cv.P3D_M2M_vel(p, p["coord"], k, 0.2)
-        d = cv.P3D_M2M_dvort(p, p, k, 0.2)
-        assert np.all(np.isfinite(v)) and np.all(np.isfinite(d))
-        assert_parity(f"vel {k}", v, orc.P3D_M2M_vel(p, p["coord"], k, 0.2), TOL)
-        assert_parity(f"dvort {k}", d, orc.P3D_M2M_dvort(p, p, k, 0.2), TOL)
-        if k in ("winckelmans", "gaussian"):
-            s = cv.P3D_M2M_visc_dvort(p, p, k, 0.2, 0.05)
-            assert_parity(f"visc {k}", s, orc.P3D_M2M_visc_dvort(p, p, k, 0.2, 0.05), TOL)
+        check(orc, k, p, p, 0.2, 0.05 if k in ("winckelmans", "gaussian") else None)
 
 
 def test_negative_radius(orc):
@@ -77,10 +86,7 @@ def test_negative_radius(orc):
     case("negative sigma")
     p = W.random_p3d(500, 41, box=1.0, vort=1.0)
     for k in ("winckelmans", "gaussian", "singular"):
-        assert_parity(f"vel {k}", cv.P3D_M2M_vel(p, p["coord"], k, -0.2),
-                      orc.P3D_M2M_vel(p, p["coord"], k, -0.2), TOL)
-        assert_parity(f"dvort {k}", cv.P3D_M2M_dvort(p, p, k, -0.2),
-                      orc.P3D_M2M_dvort(p, p, k, -0.2), TOL)
+        check(orc, k, p, p, -0.2)
 
 
 @pytest.mark.parametrize("T", [1, 2, 4])
@@ -92,10 +98,16 @@ def test_every_launch_shape(orc, T, splits):
     for k in ("winckelmans", "gaussian"):
         outs = dev.outputs(True, True, True)
         D.p3d_eval(dev.problem(outs, k, 0.1, 0.05), shape=(T, splits))
-        v, d, s = (o.download() for o in outs)
-        assert_parity(f"vel {k}", v, orc.P3D_M2M_vel(p, p["coord"], k, 0.1), TOL)
-        assert_parity(f"dvort {k}", d, orc.P3D_M2M_dvort(p, p, k, 0.1), TOL)
-        assert_parity(f"visc {k}", s, orc.P3D_M2M_visc_dvort(p, p, k, 0.1, 0.05), TOL)
+        got = [o.download() for o in outs]
+        refs = (orc.P3D_M2M_vel(p, p["coord"], k, 0.1), orc.P3D_M2M_dvort(p, p, k, 0.1),
+                orc.P3D_M2M_visc_dvort(p, p, k, 0.1, 0.05))
+        if k == "gaussian":
+            ex = orc.f64_P3D(p, p, k, 0.1, 0.05, gauss_mode=1, visc=True)
+            for n_, g, r, e in zip(("vel", "dvort", "visc"), got, refs, ex):
+                assert_parity_gaussian(f"{n_} {k}", g, r, e, TOL)
+        else:
+            for n_, g, r in zip(("vel", "dvort", "visc"), got, refs):
+                assert_parity(f"{n_} {k}", g, r, TOL)
 
 
 def test_plain_accumulation(orc):
@@ -104,8 +116,7 @@ def test_plain_accumulation(orc):
     prev = cv.set_accumulation(cv.ACC_PLAIN)
     try:
         for k in ("winckelmans", "gaussian"):
-            assert_parity(f"vel {k}", cv.P3D_M2M_vel(p, p["coord"], k, 0.1),
-                          orc.P3D_M2M_vel(p, p["coord"], k, 0.1), TOL)
+            check(orc, k, p, p, 0.1, 0.05, " (plain)")
     finally:
         cv.set_accumulation(prev)
 
@@ -135,7 +146,8 @@ def test_f3d_p2d_device_shapes(orc):
         D.f3d_eval(D.F3DProblem(dfa.addr, dfb.addr, 700, dtp.addr, dta.addr, 900, v.addr, dv.addr, 1),
                    shape=shape)
         assert_parity(f"F3D vel {shape}", v.download(), orc.F3D_M2M_vel(f, t["coord"]), TOL)
-        assert_parity(f"F3D dvort {shape}", dv.download(), orc.F3D_M2M_dvort(f, t), TOL)
+        _, d64 = orc.f64_F3D(f, t, vel=False)
+        assert_parity_limited(f"F3D dvort {shape}", dv.download(), orc.F3D_M2M_dvort(f, t), d64, TOL)
     q = W.random_p2d(800, 83, box=1.0, vort=1.0)
     dq = D.DeviceArray.from_host(D.p2d_soa(q))
     for shape in ((1, 1), (2, 3), (4, 2)):

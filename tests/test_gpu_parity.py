@@ -17,7 +17,7 @@ import pytest
 
 import cvortex_amd as cv
 from cvortex_amd import workloads as W
-from gpuutil import assert_parity, case, have_gpu, report
+from gpuutil import assert_parity, assert_parity_gaussian, assert_parity_limited, case, have_gpu, report
 import oracle
 
 pytestmark = pytest.mark.gpu
@@ -65,6 +65,12 @@ def test_p3d_vel_dvort(orc, kernel, regime, single):
             assert_parity(f"vel s{seed}", vel, v64, TOL)
             assert_parity(f"dvort s{seed}", dv, d64, TOL)
             continue
+        if kernel == "gaussian":
+            v64, d64, _ = orc.f64_P3D(p, p, kernel, SIGMA, gauss_mode=1)
+            assert_parity_gaussian(f"vel s{seed}", vel, orc.P3D_M2M_vel(p, p["coord"], kernel, SIGMA),
+                                   v64, TOL)
+            assert_parity_gaussian(f"dvort s{seed}", dv, orc.P3D_M2M_dvort(p, p, kernel, SIGMA), d64, TOL)
+            continue
         assert_parity(f"vel s{seed}", vel, orc.P3D_M2M_vel(p, p["coord"], kernel, SIGMA), TOL)
         assert_parity(f"dvort s{seed}", dv, orc.P3D_M2M_dvort(p, p, kernel, SIGMA), TOL)
 
@@ -76,7 +82,13 @@ def test_p3d_visc(orc, kernel, regime):
         p = _p3d(seed, regime)
         case(f"P3D visc {kernel} {regime} seed={seed}")
         out = cv.P3D_M2M_visc_dvort(p, p, kernel, SIGMA, NU)
-        assert_parity(f"visc s{seed}", out, orc.P3D_M2M_visc_dvort(p, p, kernel, SIGMA, NU), TOL)
+        ref = orc.P3D_M2M_visc_dvort(p, p, kernel, SIGMA, NU)
+        if kernel == "gaussian":
+            _, _, s64 = orc.f64_P3D(p, p, kernel, SIGMA, NU, gauss_mode=1, vel=False, dvort=False,
+                                    visc=True)
+            assert_parity_gaussian(f"visc s{seed}", out, ref, s64, TOL)
+        else:
+            assert_parity(f"visc s{seed}", out, ref, TOL)
 
 
 @pytest.mark.parametrize("kernel", ["singular", "planetary"])
@@ -94,10 +106,17 @@ def test_p3d_fused_equals_separate(orc, kernel):
     case(f"P3D fused {kernel}")
     visc = kernel in ("winckelmans", "gaussian")
     v, d, s = cv.P3D_M2M_fused(p, p, kernel, SIGMA, NU, vel=True, dvort=True, visc=visc)
-    assert_parity("fused vel", v, orc.P3D_M2M_vel(p, p["coord"], kernel, SIGMA), TOL)
-    assert_parity("fused dvort", d, orc.P3D_M2M_dvort(p, p, kernel, SIGMA), TOL)
+    refs = (orc.P3D_M2M_vel(p, p["coord"], kernel, SIGMA), orc.P3D_M2M_dvort(p, p, kernel, SIGMA),
+            orc.P3D_M2M_visc_dvort(p, p, kernel, SIGMA, NU) if visc else None)
+    if kernel == "gaussian":
+        ex = orc.f64_P3D(p, p, kernel, SIGMA, NU, gauss_mode=1, visc=True)
+        for name, g, r, e in zip(("fused vel", "fused dvort", "fused visc"), (v, d, s), refs, ex):
+            assert_parity_gaussian(name, g, r, e, TOL)
+        return
+    assert_parity("fused vel", v, refs[0], TOL)
+    assert_parity("fused dvort", d, refs[1], TOL)
     if visc:
-        assert_parity("fused visc", s, orc.P3D_M2M_visc_dvort(p, p, kernel, SIGMA, NU), TOL)
+        assert_parity("fused visc", s, refs[2], TOL)
 
 
 @pytest.mark.parametrize("single", [False, True], ids=["many", "single"])
@@ -115,7 +134,8 @@ def test_f3d(orc, regime, single):
         vel = cv.F3D_M2M_vel(f, p["coord"])
         dv = cv.F3D_M2M_dvort(f, p)
         assert_parity(f"vel s{seed}", vel, orc.F3D_M2M_vel(f, p["coord"]), TOL)
-        assert_parity(f"dvort s{seed}", dv, orc.F3D_M2M_dvort(f, p), TOL)
+        _, d64 = orc.f64_F3D(f, p, vel=False)
+        assert_parity_limited(f"dvort s{seed}", dv, orc.F3D_M2M_dvort(f, p), d64, TOL)
 
 
 @pytest.mark.parametrize("regime", ["spread", "dense"])
@@ -131,6 +151,9 @@ def test_p2d(orc, kernel, regime):
             # reference 1 - expf(-rho^2/2) cancels in FP32 at rho < 1e-3
             v64 = orc.f64_P2D_vel_pts(p, mes, kernel, SIGMA, gauss_mode=1)
             assert_parity(f"vel s{seed}", vel, v64, TOL)
+        elif kernel == "gaussian":
+            v64 = orc.f64_P2D_vel_pts(p, mes, kernel, SIGMA, gauss_mode=1)
+            assert_parity_gaussian(f"vel s{seed}", vel, orc.P2D_M2M_vel(p, mes, kernel, SIGMA), v64, TOL)
         else:
             assert_parity(f"vel s{seed}", vel, orc.P2D_M2M_vel(p, mes, kernel, SIGMA), TOL)
         if kernel in ("winckelmans", "gaussian"):
