@@ -1,19 +1,31 @@
 #!/usr/bin/env python3
-"""Benchmark: P3D M2M fused vel + dvort interactions/s at N = 2^20 on one GPU.
+"""Benchmarks of the gfx950 direct-sum Biot-Savart kernels on one GPU.
 
-Workload (default): the thin vortex ring of cvortex_amd.workloads.ring --
-N = 1,048,576 particles, Gaussian regularisation sigma = 0.02, nu = 1e-3.
-A step is one fused vel + dvort evaluation of all N targets against all N
-sources through the device-resident C ABI (cvtx_hip_p3d_eval), inputs
-already resident in GPU memory, compensated accumulation.  The line also
-reports the plain-FP32 and the +visc variants on the same inputs, the
-relative error against the FP64 oracle on sampled targets, the roofline of
-the pair kernel (algorithmic FP32 flops per pair x pairs / kernel time,
-against the 157.3 TFLOP/s FP32 vector peak), and the reference CPU path
-(oracle restatement, all host cores) timed on a bounded sample.
+Default (the headline metric): P3D M2M fused vel + dvort interactions/s at
+N = 2^20 -- the thin vortex ring of cvortex_amd.workloads.ring (Gaussian
+regularisation sigma = 0.02, nu = 1e-3).  A step is one fused vel + dvort
+evaluation of all N targets against all N sources through the
+device-resident C ABI (cvtx_hip_p3d_eval), inputs already resident in GPU
+memory, compensated accumulation.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
-                       [--err-samples S] [--cpu-samples M] [--no-cpu]
+Other workloads (--workload):
+  cube     N = 8192 uniform cube, Winckelmans sigma = 0.05, vel only
+  lattice  16384 straight filaments (128 x 128 sheet) on 2^20 particles,
+           F3D vel, dvort and fused
+  clouds   512 independent clouds (1024..65536 particles), fused vel + dvort,
+           one persistent launch vs one launch per cloud, RK4 steps replayed
+           from a captured HIP graph
+  sheet    N = 2^22 2D vortex sheet, planetary sigma = 4/sqrt(N), P2D vel
+           (+ the viscous term, identically zero for planetary)
+
+Each prints one JSON line with: interactions/s (pairs per second of the
+timed work), the roofline of the pair kernel (algorithmic FP32 flops per
+pair x pairs / kernel time, measured with HIP events on the kernel's
+stream, against the 157.3 TFLOP/s FP32 vector peak), the relative error
+against the FP64 oracle on sampled targets, and the reference's CPU path
+(the oracle's restatement, all host threads) timed on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ring]
 """
 import argparse
 import json
@@ -26,19 +38,391 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (packed FP32)
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak
+HBM_TBPS = 8.0
 # Algorithmic FP32 flops per pair (add/sub/mul/cmp/select 1, FMA 2,
-# transcendental 1), frozen per kernel and regularisation.
+# transcendental 1), fixed per kernel, output set and regularisation.
 FLOPS_PER_PAIR = {
+    ("p3d", "vel", "winckelmans"): 31, ("p3d", "vel", "gaussian"): 48,
     ("p3d", "vel+dvort", "singular"): 56, ("p3d", "vel+dvort", "planetary"): 58,
     ("p3d", "vel+dvort", "winckelmans"): 61, ("p3d", "vel+dvort", "gaussian"): 79,
     ("p3d", "vel+dvort+visc", "winckelmans"): 77, ("p3d", "vel+dvort+visc", "gaussian"): 94,
-    ("p3d", "vel", "winckelmans"): 31, ("p3d", "vel", "gaussian"): 48,
+    ("f3d", "vel", "singular"): 56, ("f3d", "dvort", "singular"): 76,
+    ("f3d", "vel+dvort", "singular"): 100,
+    ("p2d", "vel", "planetary"): 14,
 }
+METRIC = "P3D M2M vel+dvort interactions/s at N=1M; % FP32 VALU peak; rel err vs FP64"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def err_stats(g, r):
+    import oracle
+    rel = oracle.per_target_rel(g, r, floor=1e-30)
+    return {"max": float(rel.max()) if rel.size else 0.0,
+            "rms": float(np.sqrt(np.mean(rel * rel))) if rel.size else 0.0,
+            "rel_l2": oracle.rel_l2(g, r)}
+
+
+def roofline(pairs, F, kernel_ms):
+    achieved = pairs * F / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel_ms": kernel_ms,
+            "flops_per_pair": F}
+
+
+def traffic_for(key, n):
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(tpath):
+        return None
+    try:
+        tj = json.load(open(tpath)).get(key)
+        if tj and tj.get("n") == n:
+            return tj["bytes_per_launch"]
+    except Exception:
+        return None
+    return None
+
+
+class Timer:
+    """Wall clock and HIP events around `steps` calls of fn on `stream`."""
+
+    def __init__(self, stream):
+        self.stream = stream
+
+    def run(self, fn, steps, warmup):
+        from cvortex_amd import hip as H
+        for _ in range(warmup):
+            fn()
+        H.stream_synchronize(self.stream)
+        e0, e1 = H.Event(), H.Event()
+        w0 = time.perf_counter()
+        e0.record(self.stream)
+        for _ in range(steps):
+            fn()
+        e1.record(self.stream)
+        e1.synchronize()
+        H.stream_synchronize(self.stream)
+        wall = time.perf_counter() - w0
+        return wall / steps * 1e3, e0.elapsed_ms(e1) / steps
+
+
+def cpu_time(fn, reps=2):
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return best, out
+
+
+def base_line(args, value, unit, ms_step, config, dtype="f32"):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+            "config": config}
+
+
+# --------------------------------------------------------------------------
+# C2: ring, fused P3D vel + dvort, N = 2^20 (the headline)
+# --------------------------------------------------------------------------
+def bench_ring(args, cv, D, W, o, stream):
+    n = args.n or (1 << 20)
+    p, sigma, nu = W.ring(n)
+    dev = D.P3DDevice(p)
+    tm = Timer(stream)
+    pairs = float(n) * n
+
+    def runner(outs, acc, visc_on):
+        prob = dev.problem(outs, "gaussian", sigma, nu if visc_on else 0.0, acc)
+        return lambda: D.p3d_eval(prob, stream)
+
+    outs = dev.outputs(True, True, False)
+    log(f"ring n={n}: fused vel+dvort, compensated")
+    ms_step, ms_k = tm.run(runner(outs, cv.ACC_COMP, False), args.steps, args.warmup)
+    log(f"  {ms_step:.1f} ms/step (kernel {ms_k:.1f} ms)")
+    vel, dvort = outs[0].download(), outs[1].download()
+    F = FLOPS_PER_PAIR[("p3d", "vel+dvort", "gaussian")]
+    variants = {}
+    plain = None
+    if "plain" in args.variants:
+        o2 = dev.outputs(True, True, False)
+        s2, k2 = tm.run(runner(o2, cv.ACC_PLAIN, False), max(1, args.steps - 1), 1)
+        variants["plain_fp32"] = {"interactions_per_s": pairs / (s2 * 1e-3), "kernel_ms": k2,
+                                  "frac_of_peak": roofline(pairs, F, k2)["frac"]}
+        plain = (o2[0].download(), o2[1].download())
+    if "visc" in args.variants:
+        o3 = dev.outputs(True, True, True)
+        s3, k3 = tm.run(runner(o3, cv.ACC_COMP, True), max(1, args.steps - 1), 1)
+        F3 = FLOPS_PER_PAIR[("p3d", "vel+dvort+visc", "gaussian")]
+        variants["vel_dvort_visc"] = {"interactions_per_s": pairs / (s3 * 1e-3), "kernel_ms": k3,
+                                      "tflops": roofline(pairs, F3, k3)["achieved"],
+                                      "frac_of_peak": roofline(pairs, F3, k3)["frac"]}
+    err = None
+    if args.err_samples > 0:
+        rng = np.random.default_rng(0x5EED)
+        idx = np.sort(rng.choice(n, size=min(args.err_samples, n), replace=False)).astype(np.int32)
+        log(f"FP64 oracle on {len(idx)} sampled targets...")
+        v64, d64, _ = o.f64_P3D(p, p, "gaussian", sigma, idx=idx, gauss_mode=1)
+        err = {"samples": int(len(idx)), "vel": err_stats(vel[idx], v64),
+               "dvort": err_stats(dvort[idx], d64)}
+        if plain is not None:
+            err["plain_fp32"] = {"vel": err_stats(plain[0][idx], v64),
+                                 "dvort": err_stats(plain[1][idx], d64)}
+    cpu = None
+    if not args.no_cpu:
+        m = min(args.cpu_samples, n)
+        rng = np.random.default_rng(0x5EED + 1)
+        tidx = np.sort(rng.choice(n, size=m, replace=False))
+        tg = np.ascontiguousarray(p[tidx])
+        log(f"CPU baseline: {m} targets x {n} sources, vel then dvort")
+        best, (rv, rd) = cpu_time(lambda: (o.P3D_M2M_vel(p, tg["coord"], "gaussian", sigma),
+                                           o.P3D_M2M_dvort(p, tg, "gaussian", sigma)))
+        import oracle
+        cpu = {"value": m * float(n) / best, "unit": "interactions/s",
+               "cores": int(o.L.orc_num_threads()), "kind": "port",
+               "sample": f"{m} sampled ring targets x all {n} sources: cvtx_P3D_M2M_vel then "
+                         f"cvtx_P3D_M2M_dvort as two passes (the reference has no fused path), "
+                         f"best of 2, {best:.2f} s",
+               "rel_l2_gpu_vs_cpu": {"vel": oracle.rel_l2(vel[tidx], rv),
+                                     "dvort": oracle.rel_l2(dvort[tidx], rd)}}
+        log(f"  {cpu['value']:.3e} interactions/s on {cpu['cores']} threads")
+    line = base_line(args, pairs / (ms_step * 1e-3), "interactions/s", ms_step,
+                     {"workload": "thin vortex ring (R=1, core 0.05), fused P3D vel+dvort, "
+                                  "Gaussian sigma=0.02", "n_particles": n,
+                      "regularisation": "gaussian", "sigma": sigma,
+                      "accumulation": "compensated: FP32 tile sums + TwoSum, FP64 stretching sum"})
+    rf = roofline(pairs, F, ms_k)
+    rf["traffic"] = traffic_for("p3d_ring_vel_dvort_gaussian", n)
+    line.update({"roofline": rf, "cpu_baseline": cpu, "rel_err_vs_fp64": err, "variants": variants})
+    return line
+
+
+# --------------------------------------------------------------------------
+# C1: cube, P3D vel, N = 8192
+# --------------------------------------------------------------------------
+def bench_cube(args, cv, D, W, o, stream):
+    n = args.n or 8192
+    p, sigma = W.cube(n)
+    dev = D.P3DDevice(p)
+    tm = Timer(stream)
+    pairs = float(n) * n
+    vel = D.DeviceArray((n, 3))
+    prob = D.p3d_problem(dev.pv, dev.a, n, dev.pv, None, n, vel, None, None, "winckelmans", sigma,
+                         0.0, cv.ACC_COMP)
+    steps = max(args.steps, 50)
+    ms_step, ms_k = tm.run(lambda: D.p3d_eval(prob, stream), steps, max(args.warmup, 5))
+    V = vel.download()
+    F = FLOPS_PER_PAIR[("p3d", "vel", "winckelmans")]
+    # end to end through the public C API (gather, copies, kernels)
+    cv.P3D_M2M_vel(p, p["coord"], "winckelmans", sigma)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        Ve2e = cv.P3D_M2M_vel(p, p["coord"], "winckelmans", sigma)
+    e2e_ms = (time.perf_counter() - t0) / 10 * 1e3
+    v64 = o.f64_P3D_vel_pts(p, p["coord"], "winckelmans", sigma)
+    best, rv = cpu_time(lambda: o.P3D_M2M_vel(p, p["coord"], "winckelmans", sigma))
+    import oracle
+    algo_bytes = n * 24 + n * 16 + n * 12
+    line = base_line(args, pairs / (ms_step * 1e-3), "interactions/s", ms_step,
+                     {"workload": "uniform cube [-1,1]^3, P3D vel, Winckelmans sigma=0.05",
+                      "n_particles": n, "regularisation": "winckelmans", "sigma": sigma})
+    line["steps"] = steps
+    rf = roofline(pairs, F, ms_k)
+    rf["hbm_gbps_algorithmic"] = algo_bytes / (ms_k * 1e-3) / 1e9
+    line.update({"roofline": rf, "end_to_end_c_api_ms": e2e_ms,
+                 "rel_err_vs_fp64": {"samples": n, "vel": err_stats(V, v64),
+                                     "c_api_vel": err_stats(Ve2e, v64)},
+                 "cpu_baseline": {"value": pairs / best, "unit": "interactions/s",
+                                  "cores": int(o.L.orc_num_threads()), "kind": "port",
+                                  "sample": f"full {n} x {n} cvtx_P3D_M2M_vel, best of 2, {best:.2f} s",
+                                  "rel_l2_gpu_vs_cpu": oracle.rel_l2(V, rv)}})
+    return line
+
+
+# --------------------------------------------------------------------------
+# C3: lattice, F3D vel / dvort on 2^20 particles
+# --------------------------------------------------------------------------
+def bench_lattice(args, cv, D, W, o, stream):
+    n = args.n or (1 << 20)
+    f, t = W.lattice(n)
+    nf = len(f)
+    fa, fb = D.f3d_soa(f)
+    tpv, ta = D.p3d_soa(t)
+    dfa, dfb = D.DeviceArray.from_host(fa), D.DeviceArray.from_host(fb)
+    dtp, dta = D.DeviceArray.from_host(tpv), D.DeviceArray.from_host(ta)
+    vel, dv = D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
+    tm = Timer(stream)
+    pairs = float(nf) * n
+    res = {}
+    for name, v_, d_ in (("vel", vel, None), ("dvort", None, dv), ("vel+dvort", vel, dv)):
+        prob = D.F3DProblem(dfa.addr, dfb.addr, nf, dtp.addr, dta.addr, n,
+                            v_.addr if v_ else None, d_.addr if d_ else None, cv.ACC_COMP)
+        s, k = tm.run(lambda: D.f3d_eval(prob, stream), args.steps, args.warmup)
+        F = FLOPS_PER_PAIR[("f3d", name, "singular")]
+        res[name] = {"interactions_per_s": pairs / (s * 1e-3), "ms_per_step": s,
+                     "roofline": roofline(pairs, F, k)}
+        log(f"  F3D {name}: {s:.2f} ms")
+    V, Dv = vel.download(), dv.download()
+    rng = np.random.default_rng(0x5EED + 3)
+    idx = np.sort(rng.choice(n, size=min(args.err_samples or 4096, n), replace=False)).astype(np.int32)
+    v64, d64 = o.f64_F3D(f, t, idx=idx)
+    tg = np.ascontiguousarray(t[idx])
+    m = min(args.cpu_samples, len(idx))
+    best, (rv, rd) = cpu_time(lambda: (o.F3D_M2M_vel(f, tg["coord"][:m]), o.F3D_M2M_dvort(f, tg[:m])))
+    import oracle
+    main = res["vel+dvort"]
+    line = base_line(args, main["interactions_per_s"], "interactions/s", main["ms_per_step"],
+                     {"workload": "128x128 horseshoe sheet of straight filaments on z=0 over "
+                                  "[0,4]x[0,1], 2^20 particles in [0,6]x[-0.5,1.5]x[-0.5,0.5]",
+                      "n_filaments": nf, "n_targets": n, "outputs": "F3D vel+dvort (fused)"})
+    line.update({"roofline": main["roofline"], "separate": {k: v for k, v in res.items()},
+                 "rel_err_vs_fp64": {"samples": int(len(idx)), "vel": err_stats(V[idx], v64),
+                                     "dvort": err_stats(Dv[idx], d64)},
+                 "cpu_baseline": {"value": m * float(nf) / best, "unit": "interactions/s",
+                                  "cores": int(o.L.orc_num_threads()), "kind": "port",
+                                  "sample": f"{m} sampled targets x {nf} filaments: "
+                                            f"cvtx_F3D_M2M_vel then cvtx_F3D_M2M_dvort, best of 2",
+                                  "rel_l2_gpu_vs_cpu": {"vel": oracle.rel_l2(V[idx[:m]], rv),
+                                                        "dvort": oracle.rel_l2(Dv[idx[:m]], rd)}}})
+    return line
+
+
+# --------------------------------------------------------------------------
+# C4: 512 clouds, batched persistent launch, RK4 graph
+# --------------------------------------------------------------------------
+def bench_clouds(args, cv, D, W, o, stream):
+    n_clouds = args.n or 512
+    p, off, sig = W.clouds(n_clouds)
+    n = len(p)
+    sizes = np.diff(off)
+    pairs = float((sizes.astype(np.float64) ** 2).sum())
+    pv, a = D.p3d_soa(p)
+    dpv, da = D.DeviceArray.from_host(pv), D.DeviceArray.from_host(a)
+    vel, dv = D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
+    tm = Timer(stream)
+    b = D.Batch(off, sig, "winckelmans", outputs=cv.OUT_VEL | cv.OUT_DVORT, accumulate=cv.ACC_COMP)
+    # autotune the persistent launch shape: targets per lane x workgroups per CU
+    best = None
+    for T in (1, 2, 4):
+        for wpc in (2, 4, 8):
+            b.set_shape(T, wpc)
+            s, k = tm.run(lambda: b.eval(dpv, da, vel, dv, stream=stream), 1, 1)
+            log(f"  shape T={T} wg/cu={wpc}: {k:.1f} ms")
+            if best is None or k < best[0]:
+                best = (k, T, wpc)
+    b.set_shape(best[1], best[2])
+    ms_step, ms_k = tm.run(lambda: b.eval(dpv, da, vel, dv, stream=stream), args.steps, args.warmup)
+    V, Dv = vel.download(), dv.download()
+    v2, d2 = D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
+    pc_step, pc_k = tm.run(lambda: b.eval_per_cloud(dpv, da, v2, d2, stream=stream), args.steps, 1)
+    import oracle
+    same = {"vel": oracle.rel_l2(V, v2.download()), "dvort": oracle.rel_l2(Dv, d2.download())}
+    log(f"  persistent {ms_step:.1f} ms, per-cloud {pc_step:.1f} ms")
+    # RK4 graph
+    rk = None
+    if args.rk4_steps > 0:
+        b.rk4_init(dpv, da, 1e-3)
+        b.rk4_step(1, stream)  # warm
+        s, k = tm.run(lambda: b.rk4_step(1, stream), args.rk4_steps, 0)
+        rk = {"steps": args.rk4_steps, "ms_per_step": s, "gpu_ms_per_step": k,
+              "interactions_per_s": 4 * pairs / (s * 1e-3), "dt": 1e-3}
+        log(f"  RK4 graph step {s:.1f} ms")
+    # accuracy: 256 targets in each of 8 clouds spanning the size range
+    order = np.argsort(sizes)
+    pick = order[np.linspace(0, n_clouds - 1, 8).astype(int)]
+    rng = np.random.default_rng(0x5EED + 4)
+    errs_v, errs_d = [], []
+    for c in pick:
+        cl = np.ascontiguousarray(p[off[c]:off[c + 1]])
+        idx = np.sort(rng.choice(len(cl), size=min(256, len(cl)), replace=False)).astype(np.int32)
+        v64, d64, _ = o.f64_P3D(cl, cl, "winckelmans", float(sig[c]), idx=idx)
+        errs_v.append((V[off[c]:off[c + 1]][idx], v64))
+        errs_d.append((Dv[off[c]:off[c + 1]][idx], d64))
+    # CPU baseline: one evaluation of 8 clouds spanning the sizes up to 8192
+    small = order[sizes[order] <= 8192]
+    cpick = small[np.linspace(0, len(small) - 1, 8).astype(int)]
+    subset = [(np.ascontiguousarray(p[off[c]:off[c + 1]]), float(sig[c])) for c in cpick]
+    cpu_pairs = float(sum(len(cl) ** 2 for cl, _ in subset))
+    gv = np.concatenate([e[0] for e in errs_v]); rv = np.concatenate([e[1] for e in errs_v])
+    gd = np.concatenate([e[0] for e in errs_d]); rd = np.concatenate([e[1] for e in errs_d])
+    best_cpu, _ = cpu_time(lambda: [(o.P3D_M2M_vel(cl, cl["coord"], "winckelmans", s_),
+                                     o.P3D_M2M_dvort(cl, cl, "winckelmans", s_)) for cl, s_ in subset], 1)
+    F = FLOPS_PER_PAIR[("p3d", "vel+dvort", "winckelmans")]
+    line = base_line(args, pairs / (ms_step * 1e-3), "interactions/s", ms_step,
+                     {"workload": "512 clouds, sizes log-uniform in [1024, 65536], unit cubes, "
+                                  "Winckelmans sigma_c ~ U[0.01, 0.1], fused vel+dvort",
+                      "n_clouds": n_clouds, "n_particles": n, "pairs_per_eval": pairs,
+                      "shape": {"targets_per_lane": best[1], "workgroups_per_cu": best[2]}})
+    line.update({"roofline": roofline(pairs, F, ms_k),
+                 "per_cloud_launches": {"ms_per_step": pc_step, "interactions_per_s": pairs / (pc_step * 1e-3),
+                                        "speedup_of_persistent": pc_step / ms_step,
+                                        "rel_l2_persistent_vs_per_cloud": same},
+                 "rk4_graph": rk,
+                 "rel_err_vs_fp64": {"samples": int(len(gv)), "clouds": [int(c) for c in pick],
+                                     "vel": err_stats(gv, rv), "dvort": err_stats(gd, rd)},
+                 "cpu_baseline": {"value": cpu_pairs / best_cpu, "unit": "interactions/s",
+                                  "cores": int(o.L.orc_num_threads()), "kind": "port",
+                                  "sample": "one evaluation (vel then dvort) of 8 clouds spanning "
+                                            f"sizes {min(len(c) for c, _ in subset)}.."
+                                            f"{max(len(c) for c, _ in subset)}, {cpu_pairs:.3g} "
+                                            f"pairs, {best_cpu:.2f} s"}})
+    b.destroy()
+    return line
+
+
+# --------------------------------------------------------------------------
+# C5: sheet, P2D vel (+ visc) at N = 2^22
+# --------------------------------------------------------------------------
+def bench_sheet(args, cv, D, W, o, stream):
+    n = args.n or (1 << 22)
+    p, sigma, nu = W.sheet(n)
+    s = D.p2d_soa(p)
+    ds = D.DeviceArray.from_host(s)
+    vel, vs = D.DeviceArray((n, 2)), D.DeviceArray((n,))
+    tm = Timer(stream)
+    pairs = float(n) * n
+    prob = D.P2DProblem(ds.addr, n, ds.addr, n, vel.addr, None, 2, sigma, nu, cv.ACC_COMP)
+    ms_step, ms_k = tm.run(lambda: D.p2d_eval(prob, stream), args.steps, args.warmup)
+    V = vel.download()
+    variants = {}
+    if "plain" in args.variants:
+        v2 = D.DeviceArray((n, 2))
+        prob2 = D.P2DProblem(ds.addr, n, ds.addr, n, v2.addr, None, 2, sigma, nu, cv.ACC_PLAIN)
+        s2, k2 = tm.run(lambda: D.p2d_eval(prob2, stream), max(1, args.steps - 1), 1)
+        variants["plain_fp32"] = {"interactions_per_s": pairs / (s2 * 1e-3), "kernel_ms": k2}
+        plainV = v2.download()
+    rng = np.random.default_rng(0x5EED + 5)
+    idx = np.sort(rng.choice(n, size=min(args.err_samples or 8192, n), replace=False)).astype(np.int32)
+    v64, _ = o.f64_P2D(p, p, "planetary", sigma, idx=idx)
+    err = {"samples": int(len(idx)), "vel": err_stats(V[idx], v64)}
+    if "plain" in args.variants:
+        err["plain_fp32"] = {"vel": err_stats(plainV[idx], v64)}
+    # visc: planetary has no eta -> exactly zero (reference VortFunc.cpp:52-62)
+    vz = cv.P2D_M2M_visc_dvort(p[:4096], p[:4096], "planetary", sigma, nu)
+    m = min(args.cpu_samples, n)
+    tg = np.ascontiguousarray(p[idx[:m]])
+    best, rv = cpu_time(lambda: o.P2D_M2M_vel(p, tg["coord"], "planetary", sigma))
+    import oracle
+    F = FLOPS_PER_PAIR[("p2d", "vel", "planetary")]
+    line = base_line(args, pairs / (ms_step * 1e-3), "interactions/s", ms_step,
+                     {"workload": "2D vortex sheet y=0.01 sin(2 pi x), P2D vel, planetary "
+                                  "sigma=4/sqrt(N)", "n_particles": n, "sigma": sigma})
+    line.update({"roofline": roofline(pairs, F, ms_k), "variants": variants,
+                 "rel_err_vs_fp64": err, "visc_exactly_zero": bool(np.all(vz == 0)),
+                 "cpu_baseline": {"value": m * float(n) / best, "unit": "interactions/s",
+                                  "cores": int(o.L.orc_num_threads()), "kind": "port",
+                                  "sample": f"{m} sampled targets x {n} sources cvtx_P2D_M2M_vel, "
+                                            f"best of 2, {best:.2f} s",
+                                  "rel_l2_gpu_vs_cpu": oracle.rel_l2(V[idx[:m]], rv)}})
+    return line
+
+
+WORKLOADS = {"ring": bench_ring, "cube": bench_cube, "lattice": bench_lattice,
+             "clouds": bench_clouds, "sheet": bench_sheet}
 
 
 def main():
@@ -46,162 +430,36 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--workload", default="ring", choices=sorted(WORKLOADS))
+    ap.add_argument("--n", type=int, default=0, help="size override (clouds: number of clouds)")
     ap.add_argument("--err-samples", type=int, default=1024)
-    ap.add_argument("--cpu-samples", type=int, default=256)
+    ap.add_argument("--cpu-samples", type=int, default=512)
+    ap.add_argument("--rk4-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variants", default="plain,visc")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU only (the direct sum shards over targets "
-              "with one source all-gather per evaluation; see DESIGN.md); run with --gpus 1",
+        print("bench.py: this build measures one GPU only; the direct sum would shard its targets "
+              "over GPUs with one all-gather of the sources per evaluation. Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
     import cvortex_amd as cv
     from cvortex_amd import device as D
-    from cvortex_amd import hip as H
     from cvortex_amd import workloads as W
+    import oracle
 
     cv.initialise()
     if cv.num_accelerators() < 1:
         log("bench.py: no HIP device")
         sys.exit(1)
     log("device:", cv.accelerator_name(0))
-
-    n = args.n
-    t0 = time.time()
-    p, sigma, nu = W.ring(n)
-    log(f"workload ring n={n} built in {time.time() - t0:.1f}s")
-    dev = D.P3DDevice(p)
+    if not os.path.exists(oracle.LIB_PATH):
+        oracle.build()
+    o = oracle.default()
     stream = cv.lib().cvtx_hip_stream()
-
-    def run(outs, acc, visc_on, steps, warmup):
-        prob = dev.problem(outs, "gaussian", sigma, nu if visc_on else 0.0, acc)
-        for _ in range(warmup):
-            D.p3d_eval(prob, stream)
-        H.stream_synchronize(stream)
-        e0, e1 = H.Event(), H.Event()
-        w0 = time.perf_counter()
-        e0.record(stream)
-        for _ in range(steps):
-            D.p3d_eval(prob, stream)
-        e1.record(stream)
-        e1.synchronize()
-        H.stream_synchronize(stream)
-        wall = time.perf_counter() - w0
-        return wall / steps * 1e3, e0.elapsed_ms(e1) / steps
-
-    pairs = float(n) * float(n)
-    outs = dev.outputs(True, True, False)
-    log("timing fused vel+dvort (compensated)...")
-    ms_step, ms_kernel = run(outs, cv.ACC_COMP, False, args.steps, args.warmup)
-    log(f"  {ms_step:.1f} ms/step, kernel {ms_kernel:.1f} ms")
-    vel = outs[0].download()
-    dvort = outs[1].download()
-    F = FLOPS_PER_PAIR[("p3d", "vel+dvort", "gaussian")]
-    achieved = pairs * F / (ms_kernel * 1e-3) / 1e12
-
-    variants = {}
-    vlist = [v for v in args.variants.split(",") if v]
-    if "plain" in vlist:
-        log("timing plain FP32 accumulation...")
-        o2 = dev.outputs(True, True, False)
-        s2, k2 = run(o2, cv.ACC_PLAIN, False, max(1, args.steps - 1), 1)
-        variants["plain_fp32"] = {"interactions_per_s": pairs / (s2 * 1e-3), "kernel_ms": k2}
-        plain_vel, plain_dvort = o2[0].download(), o2[1].download()
-    if "visc" in vlist:
-        log("timing fused vel+dvort+visc...")
-        o3 = dev.outputs(True, True, True)
-        s3, k3 = run(o3, cv.ACC_COMP, True, max(1, args.steps - 1), 1)
-        F3 = FLOPS_PER_PAIR[("p3d", "vel+dvort+visc", "gaussian")]
-        variants["vel_dvort_visc"] = {"interactions_per_s": pairs / (s3 * 1e-3), "kernel_ms": k3,
-                                      "tflops": pairs * F3 / (k3 * 1e-3) / 1e12,
-                                      "frac_of_peak": pairs * F3 / (k3 * 1e-3) / 1e12 / PEAK_FP32_TFLOPS}
-
-    # ---------------- accuracy vs the FP64 oracle ----------------
-    err = None
-    cpu = None
-    if args.err_samples > 0 or not args.no_cpu:
-        import oracle
-        if not os.path.exists(oracle.LIB_PATH):
-            oracle.build()
-        o = oracle.default()
-    if args.err_samples > 0:
-        rng = np.random.default_rng(0x5EED)
-        idx = np.sort(rng.choice(n, size=min(args.err_samples, n), replace=False)).astype(np.int32)
-        log(f"FP64 oracle on {len(idx)} sampled targets...")
-        t1 = time.time()
-        v64, d64, _ = o.f64_P3D(p, p, "gaussian", sigma, idx=idx, gauss_mode=1)
-        log(f"  {time.time() - t1:.1f}s")
-
-        def stats(g, r):
-            rel = oracle.per_target_rel(g, r, floor=1e-30)
-            return {"max": float(rel.max()), "rms": float(np.sqrt(np.mean(rel * rel))),
-                    "rel_l2": oracle.rel_l2(g, r)}
-        err = {"samples": int(len(idx)), "vel": stats(vel[idx], v64), "dvort": stats(dvort[idx], d64)}
-        if "plain" in vlist:
-            err["plain_fp32"] = {"vel": stats(plain_vel[idx], v64), "dvort": stats(plain_dvort[idx], d64)}
-
-    # ---------------- CPU baseline: the reference's OpenMP path ----------------
-    if not args.no_cpu:
-        m = min(args.cpu_samples, n)
-        rng = np.random.default_rng(0x5EED + 1)
-        tidx = np.sort(rng.choice(n, size=m, replace=False))
-        tg = np.ascontiguousarray(p[tidx])
-        log(f"CPU baseline: {m} targets x {n} sources, vel then dvort...")
-        best = None
-        for rep in range(2):
-            c0 = time.perf_counter()
-            rv = o.P3D_M2M_vel(p, tg["coord"], "gaussian", sigma)
-            rd = o.P3D_M2M_dvort(p, tg, "gaussian", sigma)
-            dt = time.perf_counter() - c0
-            best = dt if best is None else min(best, dt)
-        cpu_rate = m * float(n) / best
-        cpu = {"value": cpu_rate, "unit": "interactions/s", "cores": int(o.L.orc_num_threads()),
-               "kind": "port",
-               "sample": f"{m} sampled ring targets x all {n} sources; cvtx_P3D_M2M_vel then "
-                         f"cvtx_P3D_M2M_dvort as separate passes (reference has no fused path); "
-                         f"best of 2, {best:.1f}s",
-               "rel_l2_gpu_vs_cpu": {"vel": oracle.rel_l2(vel[tidx], rv),
-                                      "dvort": oracle.rel_l2(dvort[tidx], rd)}}
-        log(f"  {cpu_rate:.3e} interactions/s on {cpu['cores']} threads")
-
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        try:
-            tj = json.load(open(tpath)).get("p3d_ring_vel_dvort_gaussian")
-            if tj and tj.get("n") == n:
-                traffic = tj["bytes_per_launch"]
-        except Exception:
-            traffic = None
-
-    line = {
-        "metric": "P3D M2M vel+dvort interactions/s at N=1M; % FP32 VALU peak; rel err vs FP64",
-        "value": pairs / (ms_step * 1e-3),
-        "unit": "interactions/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "thin vortex ring, fused P3D vel+dvort, Gaussian sigma=0.02",
-                   "n_particles": n, "regularisation": "gaussian", "sigma": sigma,
-                   "accumulation": "compensated (TwoSum tiles, FP64 epilogue)",
-                   "flops_per_pair": F},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "kernel_ms": ms_kernel},
-        "cpu_baseline": cpu,
-        "rel_err_vs_fp64": err,
-        "variants": variants,
-    }
+    line = WORKLOADS[args.workload](args, cv, D, W, o, stream)
     print(json.dumps(line), flush=True)
 
 

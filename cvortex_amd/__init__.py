@@ -90,6 +90,15 @@ _SIGNATURES = {
     "cvtx_hip_tune_save": (_I, [ctypes.c_char_p]),
     "cvtx_hip_tune_load": (_I, [ctypes.c_char_p]),
     "cvtx_hip_tune_clear": (None, []),
+    "cvtx_hip_gaussian_table": (_I, [_VP, _I]),
+    "cvtx_hip_batch_create": (_VP, [_I, _VP, _VP, _I, _I, _I, _F]),
+    "cvtx_hip_batch_destroy": (None, [_VP]),
+    "cvtx_hip_batch_set_shape": (_I, [_VP, _I, _I]),
+    "cvtx_hip_batch_info": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "cvtx_hip_batch_eval": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "cvtx_hip_batch_eval_per_cloud": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "cvtx_hip_batch_rk4_init": (_I, [_VP, _VP, _VP, _F]),
+    "cvtx_hip_batch_rk4_step": (_I, [_VP, _I, _VP]),
 }
 
 

@@ -23,267 +23,24 @@
 #include <cmath>
 
 #include "kernels.hpp"
+#include "p3d_body.hpp"
 
 namespace cvtx {
-
-// Canonical layout of the per-target sums (FP64 partials and epilogue).
-template <int MASK>
-struct P3DLayout {
-  static constexpr int VEL = 0;
-  static constexpr int A = (MASK & OUT_VEL) ? 3 : 0;       // sum Kv a_s
-  static constexpr int H = A + 3;                           // sum Kh (a_t.c) d
-  static constexpr int E = A + ((MASK & OUT_DVORT) ? 6 : 0); // sum eta a_s
-  static constexpr int EV = E + 3;                          // sum eta V_s
-  static constexpr int N = E + ((MASK & OUT_VISC) ? 4 : 0);
-  static constexpr bool NEED_TA = (MASK & (OUT_DVORT | OUT_VISC)) != 0;
-};
-
-// Register layout of the accumulators.  FP32 accumulators hold the sums
-// whose terms carry no coherent large component: vel (sum Kv d x a_s) and
-// the gradient term (sum Kh (a_t.c) d).  With ACC_COMP the stretching sum
-// sum Kv a_s and the PSE sums sum eta a_s, sum eta V_s are accumulated per
-// pair in FP64 (full rate on CDNA4): their near-field terms are large and
-// nearly parallel to a_t, and only their small component normal to a_t (or
-// the small difference V_t E - a_t e) survives, so FP32 tile sums would
-// lose several digits there.  With ACC_PLAIN everything is FP32.
-template <int MASK, int ACC>
-struct P3DRegs {
-  static constexpr bool DBL = (ACC == ACC_COMP);
-  static constexpr int F_VEL = 0;
-  static constexpr int F_A = (MASK & OUT_VEL) ? 3 : 0;                        // plain only
-  static constexpr int F_H = F_A + ((!DBL && (MASK & OUT_DVORT)) ? 3 : 0);
-  static constexpr int F_E = F_H + ((MASK & OUT_DVORT) ? 3 : 0);              // plain only
-  static constexpr int NF = F_E + ((!DBL && (MASK & OUT_VISC)) ? 4 : 0);
-  static constexpr int D_A = 0;
-  static constexpr int D_E = (MASK & OUT_DVORT) ? 3 : 0;
-  static constexpr int ND = DBL ? (D_E + ((MASK & OUT_VISC) ? 4 : 0)) : 0;
-};
 
 int p3d_nacc(int mask) {
   return ((mask & OUT_VEL) ? 3 : 0) + ((mask & OUT_DVORT) ? 6 : 0) + ((mask & OUT_VISC) ? 4 : 0);
 }
 
-struct P3DKParams {
-  const float4* __restrict__ src_pv;
-  const float4* __restrict__ src_a;
-  const float4* __restrict__ tgt_pv;
-  const float4* __restrict__ tgt_a;
-  const float4* __restrict__ gtab;
-  float* __restrict__ out_vel;
-  float* __restrict__ out_dvort;
-  float* __restrict__ out_visc;
-  double* __restrict__ partial;
-  int n_src, n_tgt, split_len, n_split;
-  RegConsts rc;
-  Scales sc;
-};
-
-template <int REG, int MASK, int ACC>
-__device__ __forceinline__ void p3d_pair(const float4 sp, const float4 sa, const double2 sxy,
-                                         const double2 szv, const float3 xt, const float3 at,
-                                         const RegConsts& rc, const float4* tab, float* acc,
-                                         double* dacc) {
-  using R = P3DRegs<MASK, ACC>;
-  const float dx = xt.x - sp.x;
-  const float dy = xt.y - sp.y;
-  const float dz = xt.z - sp.z;
-  const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-  float Kv, Kh, eta;
-  RegKernel<REG>::eval(r2, rc, tab, Kv, Kh, eta);
-  if constexpr ((MASK & (OUT_VEL | OUT_DVORT)) != 0) {
-    // c = d x a_s
-    const float cx = __builtin_fmaf(dy, sa.z, -dz * sa.y);
-    const float cy = __builtin_fmaf(dz, sa.x, -dx * sa.z);
-    const float cz = __builtin_fmaf(dx, sa.y, -dy * sa.x);
-    if constexpr ((MASK & OUT_VEL) != 0) {
-      acc[R::F_VEL + 0] = __builtin_fmaf(Kv, cx, acc[R::F_VEL + 0]);
-      acc[R::F_VEL + 1] = __builtin_fmaf(Kv, cy, acc[R::F_VEL + 1]);
-      acc[R::F_VEL + 2] = __builtin_fmaf(Kv, cz, acc[R::F_VEL + 2]);
-    }
-    if constexpr ((MASK & OUT_DVORT) != 0) {
-      if constexpr (R::DBL) {
-        const double k = (double)Kv;
-        dacc[R::D_A + 0] = __builtin_fma(k, sxy.x, dacc[R::D_A + 0]);
-        dacc[R::D_A + 1] = __builtin_fma(k, sxy.y, dacc[R::D_A + 1]);
-        dacc[R::D_A + 2] = __builtin_fma(k, szv.x, dacc[R::D_A + 2]);
-      } else {
-        acc[R::F_A + 0] = __builtin_fmaf(Kv, sa.x, acc[R::F_A + 0]);
-        acc[R::F_A + 1] = __builtin_fmaf(Kv, sa.y, acc[R::F_A + 1]);
-        acc[R::F_A + 2] = __builtin_fmaf(Kv, sa.z, acc[R::F_A + 2]);
-      }
-      const float t = Kh * __builtin_fmaf(at.x, cx, __builtin_fmaf(at.y, cy, at.z * cz));
-      acc[R::F_H + 0] = __builtin_fmaf(t, dx, acc[R::F_H + 0]);
-      acc[R::F_H + 1] = __builtin_fmaf(t, dy, acc[R::F_H + 1]);
-      acc[R::F_H + 2] = __builtin_fmaf(t, dz, acc[R::F_H + 2]);
-    }
-  }
-  if constexpr ((MASK & OUT_VISC) != 0) {
-    if constexpr (R::DBL) {
-      const double e = (double)eta;
-      dacc[R::D_E + 0] = __builtin_fma(e, sxy.x, dacc[R::D_E + 0]);
-      dacc[R::D_E + 1] = __builtin_fma(e, sxy.y, dacc[R::D_E + 1]);
-      dacc[R::D_E + 2] = __builtin_fma(e, szv.x, dacc[R::D_E + 2]);
-      dacc[R::D_E + 3] = __builtin_fma(e, szv.y, dacc[R::D_E + 3]);
-    } else {
-      acc[R::F_E + 0] = __builtin_fmaf(eta, sa.x, acc[R::F_E + 0]);
-      acc[R::F_E + 1] = __builtin_fmaf(eta, sa.y, acc[R::F_E + 1]);
-      acc[R::F_E + 2] = __builtin_fmaf(eta, sa.z, acc[R::F_E + 2]);
-      acc[R::F_E + 3] = __builtin_fmaf(eta, sp.w, acc[R::F_E + 3]);
-    }
-  }
-}
-
-// Final combination of the reduced sums for one target (FP64).
-template <int MASK>
-__device__ __forceinline__ void p3d_finish(const double* a, float3 at, float vt, const Scales& sc,
-                                           float* out_vel, float* out_dvort, float* out_visc,
-                                           int i) {
-  using L = P3DLayout<MASK>;
-  if constexpr ((MASK & OUT_VEL) != 0) {
-    out_vel[3 * i + 0] = (float)(sc.vel * a[L::VEL + 0]);
-    out_vel[3 * i + 1] = (float)(sc.vel * a[L::VEL + 1]);
-    out_vel[3 * i + 2] = (float)(sc.vel * a[L::VEL + 2]);
-  }
-  if constexpr ((MASK & OUT_DVORT) != 0) {
-    const double ax = at.x, ay = at.y, az = at.z;
-    const double Ax = a[L::A + 0], Ay = a[L::A + 1], Az = a[L::A + 2];
-    const double wx = ay * Az - az * Ay;
-    const double wy = az * Ax - ax * Az;
-    const double wz = ax * Ay - ay * Ax;
-    out_dvort[3 * i + 0] = (float)(sc.dG * wx + sc.dH * a[L::H + 0]);
-    out_dvort[3 * i + 1] = (float)(sc.dG * wy + sc.dH * a[L::H + 1]);
-    out_dvort[3 * i + 2] = (float)(sc.dG * wz + sc.dH * a[L::H + 2]);
-  }
-  if constexpr ((MASK & OUT_VISC) != 0) {
-    const double v = vt, e = a[L::EV];
-    out_visc[3 * i + 0] = (float)(sc.visc * (v * a[L::E + 0] - (double)at.x * e));
-    out_visc[3 * i + 1] = (float)(sc.visc * (v * a[L::E + 1] - (double)at.y * e));
-    out_visc[3 * i + 2] = (float)(sc.visc * (v * a[L::E + 2] - (double)at.z * e));
-  }
-}
-
 template <int REG, int MASK, int ACC, int T, int WG>
 __global__ __launch_bounds__(WG) void p3d_kernel(P3DKParams p) {
-  using L = P3DLayout<MASK>;
-  using R = P3DRegs<MASK, ACC>;
-  constexpr int NF = R::NF;
-  constexpr int ND = R::ND > 0 ? R::ND : 1;
-  __shared__ float4 s_pv[WG];
-  __shared__ float4 s_a[WG];
-  __shared__ double2 s_xy[R::ND > 0 ? WG : 1];
-  __shared__ double2 s_zv[R::ND > 0 ? WG : 1];
-  __shared__ float4 s_tab[REG == REG_GAUSSIAN ? 2 * GTAB_N : 1];
-
-  const int tid = threadIdx.x;
+  __shared__ P3DSmem<REG, MASK, ACC, WG> sm;
   if constexpr (REG == REG_GAUSSIAN) {
-    for (int i = tid; i < 2 * GTAB_N; i += WG) s_tab[i] = p.gtab[i];
+    for (int i = threadIdx.x; i < 2 * GTAB_N; i += WG) sm.tab[i] = p.gtab[i];
   }
-
-  const int base = blockIdx.x * (WG * T);
-  float3 xt[T], at[T];
-  float vt[T];
-#pragma unroll
-  for (int k = 0; k < T; ++k) {
-    int i = base + k * WG + tid;
-    i = i < p.n_tgt ? i : p.n_tgt - 1;
-    const float4 q = p.tgt_pv[i];
-    xt[k] = make_float3(q.x, q.y, q.z);
-    vt[k] = q.w;
-    if constexpr (L::NEED_TA) {
-      const float4 w = p.tgt_a[i];
-      at[k] = make_float3(w.x, w.y, w.z);
-    } else {
-      at[k] = make_float3(0.f, 0.f, 0.f);
-    }
-  }
-
-  float acc[T][NF];
-  float lo[T][NF];
-  float hi[T][NF];
-  double dacc[T][ND];
-#pragma unroll
-  for (int k = 0; k < T; ++k) {
-#pragma unroll
-    for (int a = 0; a < NF; ++a) { acc[k][a] = 0.f; lo[k][a] = 0.f; hi[k][a] = 0.f; }
-#pragma unroll
-    for (int a = 0; a < ND; ++a) dacc[k][a] = 0.0;
-  }
-
   const int s0 = blockIdx.y * p.split_len;
   const int s1 = min(s0 + p.split_len, p.n_src);
-  for (int ts = s0; ts < s1; ts += WG) {
-    const int cnt = min(WG, s1 - ts);
-    __syncthreads();
-    if (tid < cnt) {
-      const float4 q = p.src_pv[ts + tid];
-      const float4 w = p.src_a[ts + tid];
-      s_pv[tid] = q;
-      s_a[tid] = w;
-      if constexpr (R::ND > 0) {
-        s_xy[tid] = make_double2((double)w.x, (double)w.y);
-        s_zv[tid] = make_double2((double)w.z, (double)q.w);
-      }
-    }
-    __syncthreads();
-    if constexpr (ACC == ACC_COMP) {
-#pragma unroll
-      for (int k = 0; k < T; ++k)
-#pragma unroll
-        for (int a = 0; a < NF; ++a) acc[k][a] = 0.f;
-    }
-#pragma unroll 2
-    for (int j = 0; j < cnt; ++j) {
-      const float4 sp = s_pv[j];
-      const float4 sa = s_a[j];
-      double2 sxy = make_double2(0.0, 0.0), szv = make_double2(0.0, 0.0);
-      if constexpr (R::ND > 0) {
-        sxy = s_xy[j];
-        szv = s_zv[j];
-      }
-#pragma unroll
-      for (int k = 0; k < T; ++k)
-        p3d_pair<REG, MASK, ACC>(sp, sa, sxy, szv, xt[k], at[k], p.rc, s_tab, acc[k], dacc[k]);
-    }
-    if constexpr (ACC == ACC_COMP) {
-#pragma unroll
-      for (int k = 0; k < T; ++k)
-#pragma unroll
-        for (int a = 0; a < NF; ++a) two_sum_add(hi[k][a], lo[k][a], acc[k][a]);
-    }
-  }
-
-#pragma unroll
-  for (int k = 0; k < T; ++k) {
-    const int i = base + k * WG + tid;
-    if (i >= p.n_tgt) continue;
-    // gather the canonical layout
-    double d[L::N];
-    auto fval = [&](int a) -> double {
-      return (ACC == ACC_COMP) ? (double)hi[k][a] + (double)lo[k][a] : (double)acc[k][a];
-    };
-    if constexpr ((MASK & OUT_VEL) != 0) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) d[L::VEL + c] = fval(R::F_VEL + c);
-    }
-    if constexpr ((MASK & OUT_DVORT) != 0) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        d[L::A + c] = R::DBL ? dacc[k][R::D_A + c] : fval(R::F_A + c);
-        d[L::H + c] = fval(R::F_H + c);
-      }
-    }
-    if constexpr ((MASK & OUT_VISC) != 0) {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) d[L::E + c] = R::DBL ? dacc[k][R::D_E + c] : fval(R::F_E + c);
-    }
-    if (p.n_split == 1) {
-      p3d_finish<MASK>(d, at[k], vt[k], p.sc, p.out_vel, p.out_dvort, p.out_visc, i);
-    } else {
-#pragma unroll
-      for (int a = 0; a < L::N; ++a)
-        p.partial[((size_t)blockIdx.y * L::N + a) * p.n_tgt + i] = d[a];
-    }
-  }
+  p3d_block<REG, MASK, ACC, T, WG>(p, p.rc, p.sc, blockIdx.x * (WG * T), p.n_tgt, s0, s1,
+                                   blockIdx.y, sm);
 }
 
 template <int MASK>

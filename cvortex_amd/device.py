@@ -9,7 +9,7 @@ import ctypes
 import numpy as np
 
 from . import CvtxError, last_error, lib
-from .hip import DeviceArray
+from .hip import DeviceArray, synchronize  # noqa: F401  (re-exported)
 from .types import REG_ID
 
 _VP, _I, _F = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
@@ -120,3 +120,56 @@ class P3DDevice:
         v, d, s = outs
         return p3d_problem(self.pv, self.a, self.n, self.pv, self.a, self.n, v, d, s, kernel,
                            sigma, nu, accumulate)
+
+
+class Batch:
+    """Many independent P3D clouds evaluated in one persistent launch
+    (cvtx_hip_batch_*).  offsets: n+1 particle offsets, sigmas: n radii."""
+
+    def __init__(self, offsets, sigmas, kernel="winckelmans", outputs=3, accumulate=1, nu=0.0):
+        self.offsets = np.ascontiguousarray(offsets, np.int32)
+        self.sigmas = np.ascontiguousarray(sigmas, np.float32)
+        n = len(self.sigmas)
+        reg = REG_ID[kernel] if isinstance(kernel, str) else kernel
+        self.h = lib().cvtx_hip_batch_create(n, self.offsets.ctypes.data, self.sigmas.ctypes.data,
+                                             reg, outputs, accumulate, nu)
+        if not self.h:
+            raise CvtxError(f"batch_create: {last_error()}")
+
+    def _ck(self, st, what):
+        if st != 0:
+            raise CvtxError(f"{what}: {last_error()}")
+
+    def set_shape(self, targets_per_lane, workgroups_per_cu):
+        self._ck(lib().cvtx_hip_batch_set_shape(self.h, targets_per_lane, workgroups_per_cu), "set_shape")
+
+    def info(self):
+        u, g, t = _I(), _I(), _I()
+        self._ck(lib().cvtx_hip_batch_info(self.h, ctypes.byref(u), ctypes.byref(g), ctypes.byref(t)), "info")
+        return u.value, g.value, t.value
+
+    def eval(self, pv, a, vel, dvort, visc=None, stream=None):
+        self._ck(lib().cvtx_hip_batch_eval(self.h, pv.addr, a.addr, vel.addr, dvort.addr,
+                                           visc.addr if visc is not None else None, stream), "batch_eval")
+
+    def eval_per_cloud(self, pv, a, vel, dvort, visc=None, stream=None):
+        self._ck(lib().cvtx_hip_batch_eval_per_cloud(self.h, pv.addr, a.addr, vel.addr, dvort.addr,
+                                                     visc.addr if visc is not None else None, stream),
+                 "batch_eval_per_cloud")
+
+    def rk4_init(self, pv, a, dt):
+        self._ck(lib().cvtx_hip_batch_rk4_init(self.h, pv.addr, a.addr, dt), "rk4_init")
+
+    def rk4_step(self, n_steps=1, stream=None):
+        self._ck(lib().cvtx_hip_batch_rk4_step(self.h, n_steps, stream), "rk4_step")
+
+    def destroy(self):
+        if self.h:
+            lib().cvtx_hip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -176,6 +176,35 @@ int cvtx_hip_p2d_eval_shape(const cvtx_hip_p2d_problem* p, int targets_per_lane,
 int cvtx_hip_f3d_eval_shape(const cvtx_hip_f3d_problem* p, int targets_per_lane, int n_split,
                             void* stream);
 
+/* ---- batched clouds and RK4 graphs (no reference counterpart) ----
+ * A batch holds many independent P3D clouds stored back to back in one pair
+ * of device SoA buffers (pv, a as above): cloud c owns particles
+ * [offsets[c], offsets[c+1]) and has regularisation radius sigmas[c]; each
+ * cloud interacts only with itself.  `outputs` is
+ * CVTX_HIP_OUT_VEL|CVTX_HIP_OUT_DVORT (optionally |CVTX_HIP_OUT_VISC);
+ * regularisation Winckelmans or Gaussian.  offsets/sigmas are host arrays
+ * (copied).  NULL on failure (cvtx_hip_last_error). */
+typedef struct cvtx_hip_batch cvtx_hip_batch;
+cvtx_hip_batch* cvtx_hip_batch_create(int n_clouds, const int* offsets, const float* sigmas,
+                                      int regularisation, int outputs, int accumulate, float nu);
+void cvtx_hip_batch_destroy(cvtx_hip_batch* batch);
+/* Targets per lane (1, 2, 4) and resident workgroups per CU of the
+ * persistent launch; must precede cvtx_hip_batch_rk4_init. */
+int cvtx_hip_batch_set_shape(cvtx_hip_batch* batch, int targets_per_lane, int workgroups_per_cu);
+int cvtx_hip_batch_info(const cvtx_hip_batch* batch, int* n_units, int* grid, int* targets_per_lane);
+/* One persistent launch over all clouds: outputs float[3 * n_particles]. */
+int cvtx_hip_batch_eval(cvtx_hip_batch* batch, const float* pv, const float* a, float* vel,
+                        float* dvort, float* visc, void* stream);
+/* The same results with one ordinary launch per cloud (baseline). */
+int cvtx_hip_batch_eval_per_cloud(cvtx_hip_batch* batch, const float* pv, const float* a,
+                                  float* vel, float* dvort, float* visc, void* stream);
+/* Classical RK4 on (positions, vorticity) with (vel, dvort): init allocates
+ * the stage buffers and captures one whole step (4 evaluations, 4 update
+ * kernels) as a HIP graph; each step then updates pv and a in place.
+ * Requires a vel|dvort batch. */
+int cvtx_hip_batch_rk4_init(cvtx_hip_batch* batch, float* pv, float* a, float dt);
+int cvtx_hip_batch_rk4_step(cvtx_hip_batch* batch, int n_steps, void* stream);
+
 /* ---- autotuner (no reference counterpart) ----
  * Times the candidate launch shapes of a device-resident problem on the
  * active device (the problem's outputs are overwritten) and records the
