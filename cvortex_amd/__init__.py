@@ -76,6 +76,7 @@ _SIGNATURES = {
     "cvtx_hip_last_error": (ctypes.c_char_p, []),
     "cvtx_hip_set_accumulation": (_I, [_I]),
     "cvtx_hip_get_accumulation": (_I, []),
+    "cvtx_hip_set_target_order": (_I, [_I]),
     "cvtx_hip_stream": (_VP, []),
     "cvtx_hip_synchronize": (_I, []),
     # device-resident

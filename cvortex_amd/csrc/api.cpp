@@ -109,6 +109,8 @@ bool d2h(Runtime& rt, Device& d, void* dst, const void* src, size_t bytes) {
   return check_hip(rt, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, d.stream), "D2H copy");
 }
 
+bool maybe_order(Runtime& rt, Device& d, P3DJob& j, hipStream_t s);
+
 // ---------------------------- P3D host path ----------------------------
 
 int p3d_host(const cvtx_P3D** src, int ns, const bsv_V3f* mes, const cvtx_P3D** ind, int nt,
@@ -192,6 +194,7 @@ int p3d_host(const cvtx_P3D** src, int ns, const bsv_V3f* mes, const cvtx_P3D** 
     j.partial = (double*)device_buffer(rt, *d, SLOT_WORK, wb);
     if (!j.partial) return -1;
   }
+  if (!maybe_order(rt, *d, j, d->stream)) return -1;
   if (!check_hip(rt, launch_p3d(j, c, d->stream), "P3D kernel launch")) return -1;
   for (int o = 0; o < 3; ++o)
     if (d_out[o] && !d2h(rt, *d, h_out[o], d_out[o], ob)) return -1;
@@ -352,6 +355,19 @@ void report_failure(const char* fn, float* out, long n) {
 }
 
 hipStream_t pick_stream(Device* d, void* s) { return s ? (hipStream_t)s : d->stream; }
+
+// Spatially order a P3D job's targets when its kernel has a near-field-only
+// path that coherent waves can skip (see order.hip).
+bool maybe_order(Runtime& rt, Device& d, P3DJob& j, hipStream_t s) {
+  if (!rt.order_targets || !p3d_wants_order(j.reg, j.n_src, j.n_tgt)) return true;
+  const size_t wb = target_order_workspace(j.n_tgt);
+  void* ws = device_buffer(rt, d, SLOT_ORDER, wb);
+  int* perm = (int*)device_buffer(rt, d, SLOT_PERM, sizeof(int) * (size_t)j.n_tgt);
+  if (!ws || !perm) return false;
+  if (!check_hip(rt, target_order(j.tgt_pv, j.n_tgt, perm, ws, wb, s), "target order")) return false;
+  j.tgt_perm = perm;
+  return true;
+}
 
 }  // namespace
 
@@ -546,6 +562,14 @@ int cvtx_hip_set_accumulation(int mode) {
 
 int cvtx_hip_get_accumulation(void) { return runtime().acc_mode; }
 
+int cvtx_hip_set_target_order(int enable) {
+  Runtime& rt = runtime();
+  Lock lk(rt.mu);
+  const int prev = rt.order_targets ? 1 : 0;
+  rt.order_targets = enable != 0;
+  return prev;
+}
+
 void* cvtx_hip_stream(void) {
   Runtime& rt = runtime();
   Lock lk(rt.mu);
@@ -607,6 +631,7 @@ static int p3d_dev(const cvtx_hip_p3d_problem* p, const LaunchCfg* forced, void*
     j.partial = (double*)device_buffer(rt, *d, SLOT_WORK, wb);
     if (!j.partial) return -1;
   }
+  if (!maybe_order(rt, *d, j, pick_stream(d, stream))) return -1;
   return check_hip(rt, launch_p3d(j, c, pick_stream(d, stream)), "P3D launch") ? 0 : -1;
 }
 

@@ -129,8 +129,9 @@ struct RegKernel<REG_GAUSSIAN> {
   //   G = rho^-3 (1 - rho zeta),  H = (3G - zeta) rho^-2.
   // Near field (rho < 6): cubic pieces from the LDS table, built on the host
   // from exact FP64 values (series below rho = 0.5), so the small-rho
-  // behaviour has no cancellation.  Every lane evaluates both; a select
-  // picks one, so a wave never diverges.
+  // behaviour has no cancellation.  The table lookup is a branch: a wave
+  // executes it only when one of its lanes has a near pair, which for
+  // spatially ordered targets (order.hip) is a small fraction of sources.
   static __device__ __forceinline__ void eval(float r2, const RegConsts& k, const float4* tab,
                                               float& Kv, float& Kh, float& eta) {
     float q = __builtin_fmaf(r2, k.inv_s2, 1e-30f);  // rho^2; the 1e-30 keeps rho finite at r2 = 0
@@ -138,23 +139,22 @@ struct RegKernel<REG_GAUSSIAN> {
     float pinv2 = pinv * pinv;
     float e = __builtin_amdgcn_exp2f(q * k.c1);       // exp(-rho^2/2)
     float z = k.c2 * e;                               // zeta(rho)
-    float gfar = pinv2 * (pinv - z);
-    float hfar = __builtin_fmaf(3.f, gfar, -z) * pinv2;
-    // near field
-    float t = q * pinv * (float)GTAB_PER_UNIT;        // rho * 16
-    int i = (int)t;
-    i = i < (GTAB_N - 1) ? i : (GTAB_N - 1);
-    i = i > 0 ? i : 0;
-    float u = t - (float)i;
-    float4 cg = tab[2 * i];
-    float4 ch = tab[2 * i + 1];
-    float gnear = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(cg.w, u, cg.z), u, cg.y), u, cg.x);
-    float hnear = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(ch.w, u, ch.z), u, ch.y), u, ch.x);
-    bool near = q < (float)(GTAB_RHO_MAX * GTAB_RHO_MAX);
+    float g = pinv2 * (pinv - z);
+    float h = __builtin_fmaf(3.f, g, -z) * pinv2;
+    if (q < (float)(GTAB_RHO_MAX * GTAB_RHO_MAX)) {
+      float t = q * pinv * (float)GTAB_PER_UNIT;      // rho * 16
+      int i = (int)t;
+      i = i < (GTAB_N - 1) ? i : (GTAB_N - 1);
+      i = i > 0 ? i : 0;
+      float u = t - (float)i;
+      float4 cg = tab[2 * i];
+      float4 ch = tab[2 * i + 1];
+      g = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(cg.w, u, cg.z), u, cg.y), u, cg.x);
+      h = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(ch.w, u, ch.z), u, ch.y), u, ch.x);
+    }
     bool zr = (r2 == 0.f);
-    float g = near ? gnear : gfar;
     Kv = zr ? 0.f : g;
-    Kh = near ? hnear : hfar;
+    Kh = h;
     eta = zr ? 0.f : z;
   }
 };

@@ -46,6 +46,7 @@ struct P3DJob {
   float nu = 0.f;
   const float4* gtab = nullptr;  // Gaussian table (device), 2*GTAB_N float4
   double* partial = nullptr;     // workspace for n_split > 1
+  const int* tgt_perm = nullptr; // optional target order: slot i handles target tgt_perm[i]
 };
 
 struct P2DJob {
@@ -104,5 +105,17 @@ bool p2d_supported(int reg, int mask);
 
 // Gaussian near-field table (2*GTAB_N float4), computed in FP64 on the host.
 void build_gaussian_table(float* out /* 8*GTAB_N floats */);
+
+// Spatial (Morton) order of n points: perm[i] = index of the i-th point.
+// Workspace: target_order_workspace(n) bytes (device).
+size_t target_order_workspace(int n);
+hipError_t target_order(const float4* pts, int n, int* perm, void* ws, size_t ws_bytes,
+                        hipStream_t s);
+
+// Whether ordering the targets pays for a problem (a kernel with a
+// near-field-only path and enough targets to amortise the sort).
+inline bool p3d_wants_order(int reg, int n_src, int n_tgt) {
+  return reg == REG_GAUSSIAN && n_tgt >= 4096 && (long long)n_src * n_tgt >= (1LL << 26);
+}
 
 }  // namespace cvtx

@@ -55,13 +55,14 @@ __global__ __launch_bounds__(256) void p3d_reduce(P3DKParams p) {
   for (int s = 0; s < p.n_split; ++s)
 #pragma unroll
     for (int a = 0; a < NA; ++a) d[a] += p.partial[((size_t)s * NA + a) * p.n_tgt + i];
-  const float4 q = p.tgt_pv[i];
+  const int t = p.tgt_perm ? p.tgt_perm[i] : i;
+  const float4 q = p.tgt_pv[t];
   float3 at = make_float3(0.f, 0.f, 0.f);
   if constexpr (L::NEED_TA) {
-    const float4 w = p.tgt_a[i];
+    const float4 w = p.tgt_a[t];
     at = make_float3(w.x, w.y, w.z);
   }
-  p3d_finish<MASK>(d, at, q.w, p.sc, p.out_vel, p.out_dvort, p.out_visc, i);
+  p3d_finish<MASK>(d, at, q.w, p.sc, p.out_vel, p.out_dvort, p.out_visc, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -209,6 +210,7 @@ hipError_t launch_p3d(const P3DJob& j, const LaunchCfg& c, hipStream_t s) {
   p.out_dvort = j.out_dvort;
   p.out_visc = j.out_visc;
   p.partial = j.partial;
+  p.tgt_perm = j.tgt_perm;
   p.n_src = j.n_src;
   p.n_tgt = j.n_tgt;
   p.rc = p3d_consts(j.reg, j.sigma);

@@ -51,7 +51,8 @@ struct P3DKParams {
   float* __restrict__ out_vel;
   float* __restrict__ out_dvort;
   float* __restrict__ out_visc;
-  double* __restrict__ partial;
+  double* __restrict__ partial;          // indexed by target slot
+  const int* __restrict__ tgt_perm;      // slot -> target (nullptr: identity)
   int n_src, n_tgt, split_len, n_split;
   RegConsts rc;
   Scales sc;
@@ -171,6 +172,7 @@ __device__ __forceinline__ void p3d_block(const P3DKParams& p, const RegConsts& 
   for (int k = 0; k < T; ++k) {
     int i = base + k * WG + tid;
     i = i < tgt_end ? i : tgt_end - 1;
+    if (p.tgt_perm) i = p.tgt_perm[i];
     const float4 q = p.tgt_pv[i];
     xt[k] = make_float3(q.x, q.y, q.z);
     vt[k] = q.w;
@@ -260,7 +262,8 @@ __device__ __forceinline__ void p3d_block(const P3DKParams& p, const RegConsts& 
       for (int c = 0; c < 4; ++c) d[L::E + c] = R::DBL ? dacc[k][R::D_E + c] : fval(R::F_E + c);
     }
     if (p.n_split == 1) {
-      p3d_finish<MASK>(d, at[k], vt[k], sc, p.out_vel, p.out_dvort, p.out_visc, i);
+      p3d_finish<MASK>(d, at[k], vt[k], sc, p.out_vel, p.out_dvort, p.out_visc,
+                       p.tgt_perm ? p.tgt_perm[i] : i);
     } else {
 #pragma unroll
       for (int a = 0; a < L::N; ++a)

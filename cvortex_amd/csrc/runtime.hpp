@@ -28,6 +28,8 @@ enum Slot : int {
   SLOT_OUT1,
   SLOT_OUT2,
   SLOT_WORK,
+  SLOT_ORDER,  // target-order workspace
+  SLOT_PERM,   // target permutation
   SLOT_COUNT
 };
 
@@ -54,6 +56,7 @@ struct Runtime {
   std::string info;
   std::string last_error;
   int acc_mode = ACC_COMP;
+  bool order_targets = true;
   std::recursive_mutex mu;
 };
 

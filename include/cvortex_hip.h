@@ -104,6 +104,10 @@ const char* cvtx_hip_last_error(void);
  * previous mode, or -1 for an invalid mode. Default CVTX_HIP_ACC_COMP. */
 int cvtx_hip_set_accumulation(int mode);
 int cvtx_hip_get_accumulation(void);
+/* Spatial (Morton) ordering of targets before Gaussian P3D launches (on by
+ * default): waves of nearby targets skip the near-field table for distant
+ * sources.  Results do not depend on it.  Returns the previous setting. */
+int cvtx_hip_set_target_order(int enable);
 /* The library's stream on the active device (hipStream_t as void*), NULL
  * when no device is enabled. */
 void* cvtx_hip_stream(void);

@@ -121,6 +121,24 @@ def test_plain_accumulation(orc):
         cv.set_accumulation(prev)
 
 
+@pytest.mark.parametrize("n", [20000, 70000])
+def test_target_order_does_not_change_results(n):
+    # the spatial target order changes which targets share a wave, not the
+    # order in which any target's sources are summed: results are identical
+    case(f"target order on/off n={n}")
+    p = W.random_p3d(n, 91, box=1.0, vort=1.0)
+    L = cv.lib()
+    prev = L.cvtx_hip_set_target_order(1)
+    try:
+        on = cv.P3D_M2M_fused(p, p, "gaussian", 0.05, 0.01, vel=True, dvort=True, visc=True)
+        L.cvtx_hip_set_target_order(0)
+        off = cv.P3D_M2M_fused(p, p, "gaussian", 0.05, 0.01, vel=True, dvort=True, visc=True)
+    finally:
+        L.cvtx_hip_set_target_order(prev)
+    for a, b in zip(on, off):
+        assert np.array_equal(a, b)
+
+
 def test_device_api_matches_host_api():
     case("device API == host API")
     p = W.random_p3d(2000, 71, box=1.0, vort=1.0)
