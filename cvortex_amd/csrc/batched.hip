@@ -360,7 +360,7 @@ int cvtx_hip_batch_eval(cvtx_hip_batch* b, const float* pv, const float* a, floa
 // One regular launch per cloud (the unbatched baseline, same kernels and
 // accumulation, launch shape from the tuner for each cloud's size).
 int cvtx_hip_batch_eval_per_cloud(cvtx_hip_batch* b, const float* pv, const float* a, float* vel,
-                                  float* dvort, float* visc, void* stream) {
+                                  float* dvort, float* visc, int plain, void* stream) {
   Runtime& rt = runtime();
   std::lock_guard<std::recursive_mutex> lk(rt.mu);
   if (!b || (b->n_particles > 0 && (!pv || !a || !vel || !dvort))) {
@@ -391,12 +391,18 @@ int cvtx_hip_batch_eval_per_cloud(cvtx_hip_batch* b, const float* pv, const floa
     j.nu = b->nu;
     j.gtab = b->dev->gtab;
     LaunchCfg cfg = default_cfg(n, n, b->dev->n_cu);
+    if (plain) {  // one target per lane, no source partitions
+      cfg.T = 1;
+      cfg.n_split = 1;
+      cfg.split_len = 0;
+    }
     const size_t wb = p3d_workspace_bytes(j, cfg);
     if (wb) {
       j.partial = (double*)device_buffer(rt, *b->dev, SLOT_WORK, wb);
       if (!j.partial) return -1;
     }
     if (!check_hip(rt, launch_p3d(j, cfg, s), "per-cloud launch")) return -1;
+    if (plain && !check_hip(rt, hipStreamSynchronize(s), "per-cloud synchronize")) return -1;
   }
   return 0;
 }

@@ -125,37 +125,48 @@ template <>
 struct RegKernel<REG_GAUSSIAN> {
   // Works in reduced units: Kv = G(rho), Kh = H(rho) (epilogue scales by
   // |sigma|^-3 and |sigma|^-5), eta = zeta = sqrt(2/pi) exp(-rho^2/2).
-  // Far field (rho >= 6): erf(rho/sqrt2) = 1 to below 2e-9, so
-  //   G = rho^-3 (1 - rho zeta),  H = (3G - zeta) rho^-2.
-  // Near field (rho < 6): cubic pieces from the LDS table, built on the host
-  // from exact FP64 values (series below rho = 0.5), so the small-rho
-  // behaviour has no cancellation.  The table lookup is a branch: a wave
-  // executes it only when one of its lanes has a near pair, which for
-  // spatially ordered targets (order.hip) is a small fraction of sources.
+  // Three ranges of rho^2, nested so that a wave runs only the ranges its
+  // lanes need (for spatially ordered targets, order.hip, a wave's lanes
+  // are close together and most sources fall in the first range for all):
+  //   rho^2 >= 208: exp(-rho^2/2) < 2^-149 is zero in FP32, so
+  //                 G = rho^-3, H = 3 rho^-5, zeta = 0;
+  //   rho^2 >= 36:  erf(rho/sqrt2) = 1 to below 2e-9, so
+  //                 G = rho^-3 (1 - rho zeta), H = (3G - zeta) rho^-2;
+  //   rho^2 < 36:   cubic pieces from the LDS table, built on the host from
+  //                 exact FP64 values (series below rho = 0.5), so the
+  //                 small-rho behaviour has no cancellation.
+  // r2 == 0 (self or coincident pair) falls in the last range, where G and
+  // zeta are zeroed; H stays finite and only ever multiplies d = 0.
   static __device__ __forceinline__ void eval(float r2, const RegConsts& k, const float4* tab,
                                               float& Kv, float& Kh, float& eta) {
     float q = __builtin_fmaf(r2, k.inv_s2, 1e-30f);  // rho^2; the 1e-30 keeps rho finite at r2 = 0
     float pinv = __builtin_amdgcn_rsqf(q);           // 1/rho
     float pinv2 = pinv * pinv;
-    float e = __builtin_amdgcn_exp2f(q * k.c1);       // exp(-rho^2/2)
-    float z = k.c2 * e;                               // zeta(rho)
-    float g = pinv2 * (pinv - z);
-    float h = __builtin_fmaf(3.f, g, -z) * pinv2;
-    if (q < (float)(GTAB_RHO_MAX * GTAB_RHO_MAX)) {
-      float t = q * pinv * (float)GTAB_PER_UNIT;      // rho * 16
-      int i = (int)t;
-      i = i < (GTAB_N - 1) ? i : (GTAB_N - 1);
-      i = i > 0 ? i : 0;
-      float u = t - (float)i;
-      float4 cg = tab[2 * i];
-      float4 ch = tab[2 * i + 1];
-      g = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(cg.w, u, cg.z), u, cg.y), u, cg.x);
-      h = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(ch.w, u, ch.z), u, ch.y), u, ch.x);
+    float g = pinv2 * pinv;
+    float h = 3.f * g * pinv2;
+    float z = 0.f;
+    if (q < 208.f) {
+      z = k.c2 * __builtin_amdgcn_exp2f(q * k.c1);
+      g = pinv2 * (pinv - z);
+      h = __builtin_fmaf(3.f, g, -z) * pinv2;
+      if (q < (float)(GTAB_RHO_MAX * GTAB_RHO_MAX)) {
+        float t = q * pinv * (float)GTAB_PER_UNIT;  // rho * 16
+        int i = (int)t;
+        i = i < (GTAB_N - 1) ? i : (GTAB_N - 1);
+        i = i > 0 ? i : 0;
+        float u = t - (float)i;
+        float4 cg = tab[2 * i];
+        float4 ch = tab[2 * i + 1];
+        g = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(cg.w, u, cg.z), u, cg.y), u, cg.x);
+        h = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(ch.w, u, ch.z), u, ch.y), u, ch.x);
+        bool zr = (r2 == 0.f);
+        g = zr ? 0.f : g;
+        z = zr ? 0.f : z;
+      }
     }
-    bool zr = (r2 == 0.f);
-    Kv = zr ? 0.f : g;
+    Kv = g;
     Kh = h;
-    eta = zr ? 0.f : z;
+    eta = z;
   }
 };
 

@@ -152,9 +152,10 @@ class Batch:
         self._ck(lib().cvtx_hip_batch_eval(self.h, pv.addr, a.addr, vel.addr, dvort.addr,
                                            visc.addr if visc is not None else None, stream), "batch_eval")
 
-    def eval_per_cloud(self, pv, a, vel, dvort, visc=None, stream=None):
+    def eval_per_cloud(self, pv, a, vel, dvort, visc=None, stream=None, plain=False):
         self._ck(lib().cvtx_hip_batch_eval_per_cloud(self.h, pv.addr, a.addr, vel.addr, dvort.addr,
-                                                     visc.addr if visc is not None else None, stream),
+                                                     visc.addr if visc is not None else None,
+                                                     1 if plain else 0, stream),
                  "batch_eval_per_cloud")
 
     def rk4_init(self, pv, a, dt):

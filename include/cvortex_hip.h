@@ -199,9 +199,13 @@ int cvtx_hip_batch_info(const cvtx_hip_batch* batch, int* n_units, int* grid, in
 /* One persistent launch over all clouds: outputs float[3 * n_particles]. */
 int cvtx_hip_batch_eval(cvtx_hip_batch* batch, const float* pv, const float* a, float* vel,
                         float* dvort, float* visc, void* stream);
-/* The same results with one ordinary launch per cloud (baseline). */
+/* The same results with one ordinary launch per cloud (baseline).  plain = 0:
+ * each cloud gets its default launch shape (targets per lane, source
+ * partitions) and the launches are queued back to back; plain = 1: one
+ * target per lane, no source partitions, and the stream is synchronised
+ * after each cloud, as a blocking per-cloud API call would be. */
 int cvtx_hip_batch_eval_per_cloud(cvtx_hip_batch* batch, const float* pv, const float* a,
-                                  float* vel, float* dvort, float* visc, void* stream);
+                                  float* vel, float* dvort, float* visc, int plain, void* stream);
 /* Classical RK4 on (positions, vorticity) with (vel, dvort): init allocates
  * the stage buffers and captures one whole step (4 evaluations, 4 update
  * kernels) as a HIP graph; each step then updates pv and a in place.

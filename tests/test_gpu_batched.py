@@ -49,6 +49,11 @@ def test_batched_matches_per_cloud_and_oracle(orc, kernel):
     assert_parity("persistent vs per-cloud vel", V, vel2.download(), 1e-6)
     assert_parity("persistent vs per-cloud dvort", Dv, dv2.download(), 1e-6)
     assert_parity("persistent vs per-cloud visc", S, vs2.download(), 1e-6)
+    vel3, dv3, vs3 = D.DeviceArray((n, 3)), D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
+    b.eval_per_cloud(pv, a, vel3, dv3, vs3, plain=True)
+    assert_parity("persistent vs plain per-cloud vel", V, vel3.download(), 1e-6)
+    assert_parity("persistent vs plain per-cloud dvort", Dv, dv3.download(), 1e-6)
+    assert_parity("persistent vs plain per-cloud visc", S, vs3.download(), 1e-6)
     rng = np.random.default_rng(5)
     for c in range(0, len(sig), 5):
         lo_, hi_ = off[c], off[c + 1]
