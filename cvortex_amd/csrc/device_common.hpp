@@ -137,14 +137,22 @@ struct RegKernel<REG_GAUSSIAN> {
   //                 small-rho behaviour has no cancellation.
   // r2 == 0 (self or coincident pair) falls in the last range, where G and
   // zeta are zeroed; H stays finite and only ever multiplies d = 0.
-  static __device__ __forceinline__ void eval(float r2, const RegConsts& k, const float4* tab,
-                                              float& Kv, float& Kh, float& eta) {
+  //
+  // eval_ranges also reports the split between the ranges: near(Kv, eta) is
+  // called inside the rho^2 < 208 branch (only there is eta nonzero), and
+  // Kfar is Kv for the pairs beyond it and 0 for the others.  A caller can so
+  // accumulate the near pairs in a wider type at the cost of the branch only.
+  template <class Near>
+  static __device__ __forceinline__ void eval_ranges(float r2, const RegConsts& k,
+                                                     const float4* tab, float& Kv, float& Kh,
+                                                     float& eta, float& Kfar, Near&& near) {
     float q = __builtin_fmaf(r2, k.inv_s2, 1e-30f);  // rho^2; the 1e-30 keeps rho finite at r2 = 0
     float pinv = __builtin_amdgcn_rsqf(q);           // 1/rho
     float pinv2 = pinv * pinv;
     float g = pinv2 * pinv;
     float h = 3.f * g * pinv2;
     float z = 0.f;
+    float kf = g;
     if (q < 208.f) {
       z = k.c2 * __builtin_amdgcn_exp2f(q * k.c1);
       g = pinv2 * (pinv - z);
@@ -163,10 +171,19 @@ struct RegKernel<REG_GAUSSIAN> {
         g = zr ? 0.f : g;
         z = zr ? 0.f : z;
       }
+      near(g, z);
+      kf = 0.f;
     }
     Kv = g;
     Kh = h;
     eta = z;
+    Kfar = kf;
+  }
+
+  static __device__ __forceinline__ void eval(float r2, const RegConsts& k, const float4* tab,
+                                              float& Kv, float& Kh, float& eta) {
+    float kf;
+    eval_ranges(r2, k, tab, Kv, Kh, eta, kf, [](float, float) {});
   }
 };
 

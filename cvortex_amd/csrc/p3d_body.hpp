@@ -25,17 +25,22 @@ struct P3DLayout {
 // whose terms carry no coherent large component: vel (sum Kv d x a_s) and
 // the gradient term (sum Kh (a_t.c) d).  With ACC_COMP the stretching sum
 // sum Kv a_s and the PSE sums sum eta a_s, sum eta V_s are accumulated per
-// pair in FP64 (full rate on CDNA4): their near-field terms are large and
-// nearly parallel to a_t, and only their small component normal to a_t (or
-// the small difference V_t E - a_t e) survives, so FP32 tile sums would
-// lose several digits there.  With ACC_PLAIN everything is FP32.
-template <int MASK, int ACC>
+// pair in FP64: their near-field terms are large and nearly parallel to a_t,
+// and only their small component normal to a_t (or the small difference
+// V_t E - a_t e) survives, so FP32 tile sums would lose several digits
+// there.  With ACC_PLAIN everything is FP32.
+// SPLIT (Gaussian with ACC_COMP): only pairs with rho^2 < 208 go to FP64,
+// inside the regularisation's near branch; the stretching terms of the
+// pairs beyond (Kv < 208^-3/2 in reduced units, and eta = 0 there) are
+// summed in FP32 tiles combined by TwoSum, in the F_A slots.
+template <int REG, int MASK, int ACC>
 struct P3DRegs {
   static constexpr bool DBL = (ACC == ACC_COMP);
+  static constexpr bool SPLIT = DBL && REG == REG_GAUSSIAN;
   static constexpr int F_VEL = 0;
-  static constexpr int F_A = (MASK & OUT_VEL) ? 3 : 0;                        // plain only
-  static constexpr int F_H = F_A + ((!DBL && (MASK & OUT_DVORT)) ? 3 : 0);
-  static constexpr int F_E = F_H + ((MASK & OUT_DVORT) ? 3 : 0);              // plain only
+  static constexpr int F_A = (MASK & OUT_VEL) ? 3 : 0;  // plain, or the far part with SPLIT
+  static constexpr int F_H = F_A + (((!DBL || SPLIT) && (MASK & OUT_DVORT)) ? 3 : 0);
+  static constexpr int F_E = F_H + ((MASK & OUT_DVORT) ? 3 : 0);  // plain only
   static constexpr int NF = F_E + ((!DBL && (MASK & OUT_VISC)) ? 4 : 0);
   static constexpr int D_A = 0;
   static constexpr int D_E = (MASK & OUT_DVORT) ? 3 : 0;
@@ -59,17 +64,38 @@ struct P3DKParams {
 };
 
 template <int REG, int MASK, int ACC>
-__device__ __forceinline__ void p3d_pair(const float4 sp, const float4 sa, const double2 sxy,
-                                         const double2 szv, const float3 xt, const float3 at,
+__device__ __forceinline__ void p3d_pair(const float4 sp, const float4 sa, const double2* pxy,
+                                         const double2* pzv, const float3 xt, const float3 at,
                                          const RegConsts& rc, const float4* tab, float* acc,
                                          double* dacc) {
-  using R = P3DRegs<MASK, ACC>;
+  using R = P3DRegs<REG, MASK, ACC>;
   const float dx = xt.x - sp.x;
   const float dy = xt.y - sp.y;
   const float dz = xt.z - sp.z;
   const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-  float Kv, Kh, eta;
-  RegKernel<REG>::eval(r2, rc, tab, Kv, Kh, eta);
+  float Kv, Kh, eta, Kfar = 0.f;
+  if constexpr (R::SPLIT) {
+    RegKernel<REG_GAUSSIAN>::eval_ranges(r2, rc, tab, Kv, Kh, eta, Kfar, [&](float kn, float en) {
+      // FP64 source strengths are read from LDS here, on the rare path (a
+      // conversion of sa would be speculated out of the branch)
+      const double2 sxy = *pxy, szv = *pzv;
+      if constexpr ((MASK & OUT_DVORT) != 0) {
+        const double k = (double)kn;
+        dacc[R::D_A + 0] = __builtin_fma(k, sxy.x, dacc[R::D_A + 0]);
+        dacc[R::D_A + 1] = __builtin_fma(k, sxy.y, dacc[R::D_A + 1]);
+        dacc[R::D_A + 2] = __builtin_fma(k, szv.x, dacc[R::D_A + 2]);
+      }
+      if constexpr ((MASK & OUT_VISC) != 0) {
+        const double e = (double)en;
+        dacc[R::D_E + 0] = __builtin_fma(e, sxy.x, dacc[R::D_E + 0]);
+        dacc[R::D_E + 1] = __builtin_fma(e, sxy.y, dacc[R::D_E + 1]);
+        dacc[R::D_E + 2] = __builtin_fma(e, szv.x, dacc[R::D_E + 2]);
+        dacc[R::D_E + 3] = __builtin_fma(e, szv.y, dacc[R::D_E + 3]);
+      }
+    });
+  } else {
+    RegKernel<REG>::eval(r2, rc, tab, Kv, Kh, eta);
+  }
   if constexpr ((MASK & (OUT_VEL | OUT_DVORT)) != 0) {
     // c = d x a_s
     const float cx = __builtin_fmaf(dy, sa.z, -dz * sa.y);
@@ -81,7 +107,12 @@ __device__ __forceinline__ void p3d_pair(const float4 sp, const float4 sa, const
       acc[R::F_VEL + 2] = __builtin_fmaf(Kv, cz, acc[R::F_VEL + 2]);
     }
     if constexpr ((MASK & OUT_DVORT) != 0) {
-      if constexpr (R::DBL) {
+      if constexpr (R::SPLIT) {
+        acc[R::F_A + 0] = __builtin_fmaf(Kfar, sa.x, acc[R::F_A + 0]);
+        acc[R::F_A + 1] = __builtin_fmaf(Kfar, sa.y, acc[R::F_A + 1]);
+        acc[R::F_A + 2] = __builtin_fmaf(Kfar, sa.z, acc[R::F_A + 2]);
+      } else if constexpr (R::DBL) {
+        const double2 sxy = *pxy, szv = *pzv;
         const double k = (double)Kv;
         dacc[R::D_A + 0] = __builtin_fma(k, sxy.x, dacc[R::D_A + 0]);
         dacc[R::D_A + 1] = __builtin_fma(k, sxy.y, dacc[R::D_A + 1]);
@@ -97,8 +128,9 @@ __device__ __forceinline__ void p3d_pair(const float4 sp, const float4 sa, const
       acc[R::F_H + 2] = __builtin_fmaf(t, dz, acc[R::F_H + 2]);
     }
   }
-  if constexpr ((MASK & OUT_VISC) != 0) {
+  if constexpr ((MASK & OUT_VISC) != 0 && !R::SPLIT) {
     if constexpr (R::DBL) {
+      const double2 sxy = *pxy, szv = *pzv;
       const double e = (double)eta;
       dacc[R::D_E + 0] = __builtin_fma(e, sxy.x, dacc[R::D_E + 0]);
       dacc[R::D_E + 1] = __builtin_fma(e, sxy.y, dacc[R::D_E + 1]);
@@ -145,7 +177,7 @@ __device__ __forceinline__ void p3d_finish(const double* a, float3 at, float vt,
 // Shared-memory staging of one workgroup.
 template <int REG, int MASK, int ACC, int WG>
 struct P3DSmem {
-  static constexpr bool DBL = P3DRegs<MASK, ACC>::ND > 0;
+  static constexpr bool DBL = P3DRegs<REG, MASK, ACC>::ND > 0;  // FP64 source strengths
   float4 pv[WG];
   float4 a[WG];
   double2 xy[DBL ? WG : 1];
@@ -162,9 +194,10 @@ __device__ __forceinline__ void p3d_block(const P3DKParams& p, const RegConsts& 
                                           const Scales& sc, int base, int tgt_end, int s0,
                                           int s1, int split, P3DSmem<REG, MASK, ACC, WG>& sm) {
   using L = P3DLayout<MASK>;
-  using R = P3DRegs<MASK, ACC>;
+  using R = P3DRegs<REG, MASK, ACC>;
   constexpr int NF = R::NF;
   constexpr int ND = R::ND > 0 ? R::ND : 1;
+  constexpr bool DBL_ = P3DSmem<REG, MASK, ACC, WG>::DBL;
   const int tid = threadIdx.x;
   float3 xt[T], at[T];
   float vt[T];
@@ -204,7 +237,7 @@ __device__ __forceinline__ void p3d_block(const P3DKParams& p, const RegConsts& 
       const float4 w = p.src_a[ts + tid];
       sm.pv[tid] = q;
       sm.a[tid] = w;
-      if constexpr (R::ND > 0) {
+      if constexpr (DBL_) {
         sm.xy[tid] = make_double2((double)w.x, (double)w.y);
         sm.zv[tid] = make_double2((double)w.z, (double)q.w);
       }
@@ -220,14 +253,10 @@ __device__ __forceinline__ void p3d_block(const P3DKParams& p, const RegConsts& 
     for (int j = 0; j < cnt; ++j) {
       const float4 sp = sm.pv[j];
       const float4 sa = sm.a[j];
-      double2 sxy = make_double2(0.0, 0.0), szv = make_double2(0.0, 0.0);
-      if constexpr (R::ND > 0) {
-        sxy = sm.xy[j];
-        szv = sm.zv[j];
-      }
 #pragma unroll
       for (int k = 0; k < T; ++k)
-        p3d_pair<REG, MASK, ACC>(sp, sa, sxy, szv, xt[k], at[k], rc, sm.tab, acc[k], dacc[k]);
+        p3d_pair<REG, MASK, ACC>(sp, sa, &sm.xy[DBL_ ? j : 0], &sm.zv[DBL_ ? j : 0], xt[k], at[k],
+                                 rc, sm.tab, acc[k], dacc[k]);
     }
     if constexpr (ACC == ACC_COMP) {
 #pragma unroll
@@ -253,7 +282,10 @@ __device__ __forceinline__ void p3d_block(const P3DKParams& p, const RegConsts& 
     if constexpr ((MASK & OUT_DVORT) != 0) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        d[L::A + c] = R::DBL ? dacc[k][R::D_A + c] : fval(R::F_A + c);
+        if constexpr (R::SPLIT)
+          d[L::A + c] = dacc[k][R::D_A + c] + fval(R::F_A + c);
+        else
+          d[L::A + c] = R::DBL ? dacc[k][R::D_A + c] : fval(R::F_A + c);
         d[L::H + c] = fval(R::F_H + c);
       }
     }
