@@ -123,16 +123,16 @@ struct RegKernel<REG_PLANETARY> {
 
 template <>
 struct RegKernel<REG_GAUSSIAN> {
-  // Works in reduced units: Kv = G(rho), Kh = H(rho) (epilogue scales by
-  // |sigma|^-3 and |sigma|^-5), eta = zeta = sqrt(2/pi) exp(-rho^2/2).
+  // Works in reduced units: Kv = G(rho), Kh = H(rho)/3 (the epilogue scales
+  // by |sigma|^-3 and 3 |sigma|^-5), eta = zeta = sqrt(2/pi) exp(-rho^2/2).
   // Three ranges of rho^2, nested so that a wave runs only the ranges its
   // lanes need (for spatially ordered targets, order.hip, a wave's lanes
   // are close together and most sources fall in the first range for all):
   //   rho^2 >= 208: exp(-rho^2/2) < 2^-149 is zero in FP32, so
-  //                 G = rho^-3, H = 3 rho^-5, zeta = 0;
+  //                 G = rho^-3, H/3 = rho^-5, zeta = 0;
   //   rho^2 >= 36:  erf(rho/sqrt2) = 1 to below 2e-9, so
-  //                 G = rho^-3 (1 - rho zeta), H = (3G - zeta) rho^-2;
-  //   rho^2 < 36:   cubic pieces from the LDS table, built on the host from
+  //                 G = rho^-3 (1 - rho zeta), H/3 = (G - zeta/3) rho^-2;
+  //   rho^2 < 36:   cubic pieces of G and H/3 from the LDS table, built on the host from
   //                 exact FP64 values (series below rho = 0.5), so the
   //                 small-rho behaviour has no cancellation.
   // r2 == 0 (self or coincident pair) falls in the last range, where G and
@@ -150,13 +150,13 @@ struct RegKernel<REG_GAUSSIAN> {
     float pinv = __builtin_amdgcn_rsqf(q);           // 1/rho
     float pinv2 = pinv * pinv;
     float g = pinv2 * pinv;
-    float h = 3.f * g * pinv2;
+    float h = g * pinv2;
     float z = 0.f;
     float kf = g;
     if (q < 208.f) {
       z = k.c2 * __builtin_amdgcn_exp2f(q * k.c1);
       g = pinv2 * (pinv - z);
-      h = __builtin_fmaf(3.f, g, -z) * pinv2;
+      h = __builtin_fmaf(z, -1.f / 3.f, g) * pinv2;
       if (q < (float)(GTAB_RHO_MAX * GTAB_RHO_MAX)) {
         float t = q * pinv * (float)GTAB_PER_UNIT;  // rho * 16
         int i = (int)t;

@@ -75,6 +75,7 @@ void build_gaussian_table(float* out) {
     for (int j = 0; j < 4; ++j) {
       const double rho = (i + u[j]) / (double)GTAB_PER_UNIT;
       gaussian_GH(rho, &fg[j], &fh[j]);
+      fh[j] /= 3.0;  // the kernels use H/3 (the factor 3 is in the epilogue scale)
     }
     cubic_through(u, fg, cg);
     cubic_through(u, fh, ch);

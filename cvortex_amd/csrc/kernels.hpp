@@ -103,7 +103,8 @@ RegConsts p3d_consts(int reg, float sigma);
 bool p3d_supported(int reg, int mask);
 bool p2d_supported(int reg, int mask);
 
-// Gaussian near-field table (2*GTAB_N float4), computed in FP64 on the host.
+// Gaussian near-field table (2*GTAB_N float4): cubic pieces of G and H/3,
+// computed in FP64 on the host.
 void build_gaussian_table(float* out /* 8*GTAB_N floats */);
 
 // Spatial (Morton) order of n points: perm[i] = index of the i-th point.

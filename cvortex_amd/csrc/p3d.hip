@@ -98,7 +98,7 @@ Scales p3d_scales(int reg, float sigma, float nu) {
     case REG_GAUSSIAN:
       sc.vel = -k4pi / s3;
       sc.dG = sgn * k4pi / s3;
-      sc.dH = sgn * k4pi / s5;
+      sc.dH = 3.0 * sgn * k4pi / s5;
       sc.visc = 2.0 * (double)nu / (s * s);
       break;
     case REG_WINCKELMANS:

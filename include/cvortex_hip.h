@@ -227,8 +227,8 @@ void cvtx_hip_tune_clear(void);
 
 /* ---- inspection (no reference counterpart) ----
  * Copies the Gaussian near-field table used by the kernels (96 pieces x
- * 8 floats: cubic coefficients of G = g/rho^3 and H = (3G - zeta)/rho^2 on
- * rho in [i/16, (i+1)/16)) into `out`; returns the number of floats. */
+ * 8 floats: cubic coefficients of G = g/rho^3 and H/3 = (G - zeta/3)/rho^2
+ * on rho in [i/16, (i+1)/16)) into `out`; returns the number of floats. */
 int cvtx_hip_gaussian_table(float* out, int capacity);
 
 #ifdef __cplusplus

@@ -1,7 +1,7 @@
 """The Gaussian near-field table used by the kernels is accurate.
 
-The table holds cubic pieces of G(rho) = g(rho)/rho^3 and
-H(rho) = (3G - zeta)/rho^2 on [0, 6).  Both are integrals,
+The table holds cubic pieces of G(rho) = g(rho)/rho^3 and of H(rho)/3, with
+H(rho) = (3G - zeta)/rho^2, on [0, 6).  G and H are integrals,
 G = sqrt(2/pi) int_0^1 s^2 exp(-rho^2 s^2 / 2) ds and H the same with s^4,
 evaluated here by mpmath quadrature -- independently of the library's
 series/erf construction and of the oracle."""
@@ -31,7 +31,7 @@ def test_table_accuracy():
         u = rho * 16 - i
         cg, ch = tab[i]
         G = ((cg[3] * u + cg[2]) * u + cg[1]) * u + cg[0]
-        H = ((ch[3] * u + ch[2]) * u + ch[1]) * u + ch[0]
+        H = 3.0 * (((ch[3] * u + ch[2]) * u + ch[1]) * u + ch[0])
         Ge, He = exact(rho)
         worst = max(worst, abs(G - Ge) / Ge, abs(H - He) / He)
     assert worst < 2e-7, worst
