@@ -93,20 +93,29 @@ class Timer:
         self.stream = stream
 
     def run(self, fn, steps, warmup):
+        """(wall ms per step, event ms per step) over `steps` calls after
+        `warmup` untimed ones; self.last_steps_ms holds each step's event
+        time (an event is recorded between consecutive steps)."""
         from cvortex_amd import hip as H
         for _ in range(warmup):
             fn()
         H.stream_synchronize(self.stream)
-        e0, e1 = H.Event(), H.Event()
+        ev = [H.Event() for _ in range(steps + 1)]
         w0 = time.perf_counter()
-        e0.record(self.stream)
-        for _ in range(steps):
+        ev[0].record(self.stream)
+        for k in range(steps):
             fn()
-        e1.record(self.stream)
-        e1.synchronize()
+            ev[k + 1].record(self.stream)
+        ev[-1].synchronize()
         H.stream_synchronize(self.stream)
         wall = time.perf_counter() - w0
-        return wall / steps * 1e3, e0.elapsed_ms(e1) / steps
+        self.last_steps_ms = [ev[k].elapsed_ms(ev[k + 1]) for k in range(steps)]
+        return wall / steps * 1e3, ev[0].elapsed_ms(ev[-1]) / steps
+
+
+def n_err(args, default):
+    """Error-sample count: the workload's default unless --err-samples is given."""
+    return default if args.err_samples < 0 else args.err_samples
 
 
 def cpu_time(fn, reps=2):
@@ -143,7 +152,8 @@ def bench_ring(args, cv, D, W, o, stream):
     outs = dev.outputs(True, True, False)
     log(f"ring n={n}: fused vel+dvort, compensated")
     ms_step, ms_k = tm.run(runner(outs, cv.ACC_COMP, False), args.steps, args.warmup)
-    log(f"  {ms_step:.1f} ms/step (kernel {ms_k:.1f} ms)")
+    steps_ms = list(tm.last_steps_ms)
+    log(f"  {ms_step:.1f} ms/step (kernel {ms_k:.1f} ms, median {np.median(steps_ms):.1f} ms)")
     vel, dvort = outs[0].download(), outs[1].download()
     F = FLOPS_PER_PAIR[("p3d", "vel+dvort", "gaussian")]
     variants = {}
@@ -162,9 +172,9 @@ def bench_ring(args, cv, D, W, o, stream):
                                       "tflops": roofline(pairs, F3, k3)["achieved"],
                                       "frac_of_peak": roofline(pairs, F3, k3)["frac"]}
     err = None
-    if args.err_samples > 0:
+    if n_err(args, 4096) > 0:
         rng = np.random.default_rng(0x5EED)
-        idx = np.sort(rng.choice(n, size=min(args.err_samples, n), replace=False)).astype(np.int32)
+        idx = np.sort(rng.choice(n, size=min(n_err(args, 4096), n), replace=False)).astype(np.int32)
         log(f"FP64 oracle on {len(idx)} sampled targets...")
         v64, d64, _ = o.f64_P3D(p, p, "gaussian", sigma, idx=idx, gauss_mode=1)
         err = {"samples": int(len(idx)), "vel": err_stats(vel[idx], v64),
@@ -194,8 +204,11 @@ def bench_ring(args, cv, D, W, o, stream):
                      {"workload": "thin vortex ring (R=1, core 0.05), fused P3D vel+dvort, "
                                   "Gaussian sigma=0.02", "n_particles": n,
                       "regularisation": "gaussian", "sigma": sigma,
-                      "accumulation": "compensated: FP32 tile sums + TwoSum, FP64 stretching sum"})
+                      "accumulation": "compensated: FP32 tile sums + TwoSum; FP64 stretching "
+                                      "sums for pairs within rho = 14.4; FP64 epilogue"})
     rf = roofline(pairs, F, ms_k)
+    rf["kernel_ms_per_step"] = [round(t, 3) for t in steps_ms]
+    rf["kernel_ms_median"] = float(np.median(steps_ms))
     rf["traffic"] = traffic_for("p3d_ring_vel_dvort_gaussian", n)
     line.update({"roofline": rf, "cpu_baseline": cpu, "rel_err_vs_fp64": err, "variants": variants})
     return line
@@ -268,7 +281,7 @@ def bench_lattice(args, cv, D, W, o, stream):
         log(f"  F3D {name}: {s:.2f} ms")
     V, Dv = vel.download(), dv.download()
     rng = np.random.default_rng(0x5EED + 3)
-    idx = np.sort(rng.choice(n, size=min(args.err_samples or 4096, n), replace=False)).astype(np.int32)
+    idx = np.sort(rng.choice(n, size=min(n_err(args, 4096) or 4096, n), replace=False)).astype(np.int32)
     v64, d64 = o.f64_F3D(f, t, idx=idx)
     tg = np.ascontiguousarray(t[idx])
     m = min(args.cpu_samples, len(idx))
@@ -409,7 +422,7 @@ def bench_sheet(args, cv, D, W, o, stream):
         variants["plain_fp32"] = {"interactions_per_s": pairs / (s2 * 1e-3), "kernel_ms": k2}
         plainV = v2.download()
     rng = np.random.default_rng(0x5EED + 5)
-    idx = np.sort(rng.choice(n, size=min(args.err_samples or 8192, n), replace=False)).astype(np.int32)
+    idx = np.sort(rng.choice(n, size=min(n_err(args, 8192) or 8192, n), replace=False)).astype(np.int32)
     v64, _ = o.f64_P2D(p, p, "planetary", sigma, idx=idx)
     err = {"samples": int(len(idx)), "vel": err_stats(V[idx], v64)}
     if "plain" in args.variants:
@@ -441,11 +454,12 @@ WORKLOADS = {"ring": bench_ring, "cube": bench_cube, "lattice": bench_lattice,
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="ring", choices=sorted(WORKLOADS))
     ap.add_argument("--n", type=int, default=0, help="size override (clouds: number of clouds)")
-    ap.add_argument("--err-samples", type=int, default=1024)
+    ap.add_argument("--err-samples", type=int, default=-1,
+                    help="targets checked against the FP64 oracle (default: per workload)")
     ap.add_argument("--cpu-samples", type=int, default=512)
     ap.add_argument("--rk4-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")

@@ -34,7 +34,8 @@ struct Reg2D<REG_SINGULAR> {
 };
 
 // Winckelmans: a = q + 1, y = 1/a
-//   g_2D / r^2 = (q + 2) / ((q + 1)^2 sigma^2)  -> K = (a + 1) y^2  (x 1/sigma^2)
+//   g_2D / r^2 = (q + 2) / ((q + 1)^2 sigma^2) = (y + y^2) / sigma^2
+//            -> K = y + y^2  (x 1/sigma^2)
 //   eta_2D = 24 exp(4 / a^3) / a^4  (reference VortFunc.cpp:124-131, kept as is)
 //            -> eta = exp(4 y^3) y^4  (x 24)
 template <>
@@ -42,21 +43,22 @@ struct Reg2D<REG_WINCKELMANS> {
   static __device__ __forceinline__ void eval(float r2, const RegConsts& k, float& K, float& eta) {
     float a = __builtin_fmaf(r2, k.inv_s2, 1.f);
     float y = __builtin_amdgcn_rcpf(a);
+    K = __builtin_fmaf(y, y, y);  // finite at r2 = 0 and multiplied by d = 0 there
     float y2 = y * y;
-    K = (a + 1.f) * y2;  // finite at r2 = 0 and multiplied by d = 0 there
     float e = __builtin_amdgcn_exp2f((y2 * y) * k.c1);
     float et = e * (y2 * y2);
     eta = (r2 == 0.f) ? 0.f : et;
   }
 };
 
-// planetary: g = rho^2 inside the core, 1 outside -> K = 1 or 1/q (x 1/sigma^2)
+// planetary: g = rho^2 inside the core, 1 outside, so g / r^2 is 1/sigma^2
+// inside and 1/r^2 outside: K = 1 / max(r^2, sigma^2) (dimensional).  The
+// host floors sigma^2 at 1e-30, so K stays finite for a coincident pair
+// (where it multiplies d = 0) also when sigma = 0.
 template <>
 struct Reg2D<REG_PLANETARY> {
   static __device__ __forceinline__ void eval(float r2, const RegConsts& k, float& K, float& eta) {
-    float q = r2 * k.inv_s2;
-    float iq = __builtin_amdgcn_rcpf(q);
-    K = (q < 1.f) ? 1.f : iq;
+    K = __builtin_amdgcn_rcpf(__builtin_fmaxf(r2, k.s2));
     eta = 0.f;
   }
 };
@@ -289,7 +291,8 @@ hipError_t launch_p2d(const P2DJob& j, const LaunchCfg& c, hipStream_t s) {
       p.sc_visc = 2.0 * (double)j.nu / (sa * sa) * 24.0;
       break;
     case REG_PLANETARY:
-      p.sc_vel = k2pi / (sa * sa);
+      p.rc.s2 = std::fmax((float)(sa * sa), 1e-30f);
+      p.sc_vel = k2pi;
       break;
     default:
       p.rc.c1 = (float)(-0.5 / std::log(2.0));
