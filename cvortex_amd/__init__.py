@@ -55,6 +55,8 @@ _SIGNATURES = {
     "cvtx_VortFunc_gaussian": (VortFunc, []),
     # public M2M (void)
     "cvtx_P3D_M2M_vel": (None, [_VP, _I, _VP, _I, _VP, _VP, _F]),
+    "cvtx_P3D_M2M_vort": (None, [_VP, _I, _VP, _I, _VP, _VP, _F]),
+    "cvtx_P3D_pedrizzetti_relaxation": (None, [_VP, _I, _F, _VP, _F]),
     "cvtx_P3D_M2M_dvort": (None, [_VP, _I, _VP, _I, _VP, _VP, _F]),
     "cvtx_P3D_M2M_visc_dvort": (None, [_VP, _I, _VP, _I, _VP, _VP, _F, _F]),
     "cvtx_F3D_M2M_vel": (None, [_VP, _I, _VP, _I, _VP]),
@@ -65,6 +67,8 @@ _SIGNATURES = {
     "cvtx_hip_P3D_M2M_vel": (_I, [_VP, _I, _VP, _I, _VP, _VP, _F]),
     "cvtx_hip_P3D_M2M_dvort": (_I, [_VP, _I, _VP, _I, _VP, _VP, _F]),
     "cvtx_hip_P3D_M2M_visc_dvort": (_I, [_VP, _I, _VP, _I, _VP, _VP, _F, _F]),
+    "cvtx_hip_P3D_M2M_vort": (_I, [_VP, _I, _VP, _I, _VP, _VP, _F]),
+    "cvtx_hip_P3D_pedrizzetti_relaxation": (_I, [_VP, _I, _F, _VP, _F]),
     "cvtx_hip_F3D_M2M_vel": (_I, [_VP, _I, _VP, _I, _VP]),
     "cvtx_hip_F3D_M2M_dvort": (_I, [_VP, _I, _VP, _I, _VP]),
     "cvtx_hip_P2D_M2M_vel": (_I, [_VP, _I, _VP, _I, _VP, _VP, _F]),
@@ -81,6 +85,8 @@ _SIGNATURES = {
     "cvtx_hip_synchronize": (_I, []),
     # device-resident
     "cvtx_hip_p3d_eval": (_I, [_VP, _VP]),
+    "cvtx_hip_vort_eval": (_I, [_VP, _VP]),
+    "cvtx_hip_relax_eval": (_I, [_VP, _VP, _I, _F, _I, _F, _I, _VP]),
     "cvtx_hip_p2d_eval": (_I, [_VP, _VP]),
     "cvtx_hip_f3d_eval": (_I, [_VP, _VP]),
     "cvtx_hip_p3d_eval_shape": (_I, [_VP, _I, _I, _VP]),
@@ -231,6 +237,30 @@ def P3D_M2M_vel(particles, mes, kernel, regularisation_radius):
     _check(lib().cvtx_hip_P3D_M2M_vel(_vp(p), len(p), _vp(mes), mes.shape[0], _vp(out), kp,
                                       float(regularisation_radius)), "P3D_M2M_vel")
     return out
+
+
+def P3D_M2M_vort(particles, mes, kernel, regularisation_radius):
+    """Vorticity field of 3D vortex particles at points mes (n,3), with the
+    reference CPU path's 5 sigma box cut-off.  libcvtx.h:241-248."""
+    p, keep = _ptrs(particles, P3D_DTYPE)
+    mes = np.ascontiguousarray(mes, dtype=np.float32).reshape(-1, 3)
+    out = np.empty((mes.shape[0], 3), np.float32)
+    k, kp = _kref(kernel)
+    _check(lib().cvtx_hip_P3D_M2M_vort(_vp(p), len(p), _vp(mes), mes.shape[0], _vp(out), kp,
+                                       float(regularisation_radius)), "P3D_M2M_vort")
+    return out
+
+
+def P3D_pedrizzetti_relaxation(particles, fdt, kernel, regularisation_radius):
+    """Pedrizzetti relaxation (libcvtx.h:259-264); returns a relaxed copy of
+    the records (the library updates the records it is given in place)."""
+    recs = np.array(particles, dtype=P3D_DTYPE, copy=True)
+    p, keep = _ptrs(recs, P3D_DTYPE)
+    k, kp = _kref(kernel)
+    _check(lib().cvtx_hip_P3D_pedrizzetti_relaxation(_vp(p), len(recs), float(fdt), kp,
+                                                     float(regularisation_radius)),
+           "P3D_pedrizzetti_relaxation")
+    return recs
 
 
 def P3D_M2M_dvort(particles, induced, kernel, regularisation_radius):

@@ -20,13 +20,12 @@
 
 #include "kernels.hpp"
 #include "runtime.hpp"
+#include "api_common.hpp"
 #include "tune.hpp"
 
 using namespace cvtx;
 
-namespace {
-
-using Lock = std::lock_guard<std::recursive_mutex>;
+namespace cvtx {
 
 // ---------------- host gathers: AoP records -> SoA float4 ----------------
 
@@ -108,6 +107,29 @@ bool d2h(Runtime& rt, Device& d, void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return true;
   return check_hip(rt, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, d.stream), "D2H copy");
 }
+
+void report_failure(const char* fn, float* out, long n) {
+  std::fprintf(stderr, "cvortex: %s failed: %s (result set to NaN)\n", fn,
+               runtime().last_error.c_str());
+  fill_nan(out, n);
+}
+
+hipStream_t pick_stream(Device* d, void* s) { return s ? (hipStream_t)s : d->stream; }
+
+int* order_points(Runtime& rt, Device& d, const float4* pts, int n, int slot, hipStream_t s) {
+  const size_t wb = target_order_workspace(n);
+  void* ws = device_buffer(rt, d, SLOT_ORDER, wb);
+  int* perm = (int*)device_buffer(rt, d, slot, sizeof(int) * (size_t)n);
+  if (!ws || !perm) return nullptr;
+  if (!check_hip(rt, target_order(pts, n, perm, ws, wb, s), "spatial order")) return nullptr;
+  return perm;
+}
+
+}  // namespace cvtx
+
+namespace {
+
+using Lock = std::lock_guard<std::recursive_mutex>;
 
 bool maybe_order(Runtime& rt, Device& d, P3DJob& j, hipStream_t s);
 
@@ -348,25 +370,12 @@ int p2d_host(const cvtx_P2D** src, int ns, const bsv_V2f* mes, const cvtx_P2D** 
   return 0;
 }
 
-void report_failure(const char* fn, float* out, long n) {
-  std::fprintf(stderr, "cvortex: %s failed: %s (result set to NaN)\n", fn,
-               runtime().last_error.c_str());
-  fill_nan(out, n);
-}
-
-hipStream_t pick_stream(Device* d, void* s) { return s ? (hipStream_t)s : d->stream; }
-
 // Spatially order a P3D job's targets when its kernel has a near-field-only
 // path that coherent waves can skip (see order.hip).
 bool maybe_order(Runtime& rt, Device& d, P3DJob& j, hipStream_t s) {
   if (!rt.order_targets || !p3d_wants_order(j.reg, j.n_src, j.n_tgt)) return true;
-  const size_t wb = target_order_workspace(j.n_tgt);
-  void* ws = device_buffer(rt, d, SLOT_ORDER, wb);
-  int* perm = (int*)device_buffer(rt, d, SLOT_PERM, sizeof(int) * (size_t)j.n_tgt);
-  if (!ws || !perm) return false;
-  if (!check_hip(rt, target_order(j.tgt_pv, j.n_tgt, perm, ws, wb, s), "target order")) return false;
-  j.tgt_perm = perm;
-  return true;
+  j.tgt_perm = order_points(rt, d, j.tgt_pv, j.n_tgt, SLOT_PERM, s);
+  return j.tgt_perm != nullptr;
 }
 
 }  // namespace

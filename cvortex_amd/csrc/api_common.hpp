@@ -1,0 +1,35 @@
+// api_common.hpp -- host staging helpers shared by the C-ABI translation
+// units (api.cpp, vort_api.cpp): gathers of the caller's pointer arrays into
+// pinned structure-of-arrays buffers, copies on the device stream, failure
+// reporting and spatial ordering.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cvortex/libcvtx.h>
+
+#include "runtime.hpp"
+
+namespace cvtx {
+
+// AoP records -> SoA float4 (parallel over host threads)
+void gather_p3d(const cvtx_P3D* const* src, int n, float4* pv, float4* a);
+void gather_pts3(const bsv_V3f* pts, int n, float4* pv);
+void gather_p2d(const cvtx_P2D* const* src, int n, float4* out);
+void gather_pts2(const bsv_V2f* pts, int n, float4* out);
+void gather_f3d(const cvtx_F3D* const* src, int n, float4* fa, float4* fb);
+
+void fill_nan(float* out, long n);
+void warn_no_eta_once();
+// true (and the error recorded) for negative counts or missing arrays
+bool bad_counts(Runtime& rt, int ns, int nt, const void* src, const void* tgt);
+// asynchronous copies on the device's stream
+bool h2d(Runtime& rt, Device& d, void* dst, const void* src, size_t bytes);
+bool d2h(Runtime& rt, Device& d, void* dst, const void* src, size_t bytes);
+// stderr message with the last error, and NaN results
+void report_failure(const char* fn, float* out, long n);
+hipStream_t pick_stream(Device* d, void* s);
+// Morton permutation of n points into runtime slot `slot` (nullptr on failure)
+int* order_points(Runtime& rt, Device& d, const float4* pts, int n, int slot, hipStream_t s);
+
+}  // namespace cvtx

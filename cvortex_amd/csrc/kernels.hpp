@@ -78,6 +78,21 @@ struct F3DJob {
   double* partial = nullptr;
 };
 
+// Vorticity field (cvtx_P3D_M2M_vort semantics: 5 sigma box cut-off).
+struct VortJob {
+  const float4* src_pv = nullptr;  // {x, y, z, volume}
+  const float4* src_a = nullptr;   // {alpha, 0}
+  const int* src_perm = nullptr;   // Morton order of the sources: enables tile skipping
+  int n_src = 0;
+  const float4* tgt = nullptr;     // {x, y, z, -}
+  const int* tgt_perm = nullptr;   // Morton order of the targets (nullptr: identity)
+  int n_tgt = 0;
+  float* out = nullptr;            // float[3 n_tgt]
+  int reg = REG_WINCKELMANS;
+  int acc = ACC_COMP;
+  float sigma = 1.f;
+};
+
 // number of FP64 partial accumulators per target for a mask
 int p3d_nacc(int mask);
 int p2d_nacc(int mask);
@@ -96,6 +111,14 @@ LaunchCfg default_cfg(int n_src, int n_tgt, int n_cu);
 hipError_t launch_p3d(const P3DJob& j, const LaunchCfg& c, hipStream_t s);
 hipError_t launch_p2d(const P2DJob& j, const LaunchCfg& c, hipStream_t s);
 hipError_t launch_f3d(const F3DJob& j, const LaunchCfg& c, hipStream_t s);
+// Vorticity field: workspace of vort_workspace_bytes (source partitions;
+// sorted source copies and tile boxes when j.src_perm is set).
+size_t vort_workspace_bytes(const VortJob& j, const LaunchCfg& c, bool sorted);
+hipError_t launch_vort(const VortJob& j, const LaunchCfg& c, void* ws, size_t ws_bytes,
+                       hipStream_t s);
+double vort_scale(int reg, float sigma);
+// Pedrizzetti update of a[n] (float4 {alpha, w}) from omega[3 n], in place.
+hipError_t launch_relax(float4* a, const float* omega, int n, float fdt, hipStream_t s);
 
 // Host-side helpers.
 Scales p3d_scales(int reg, float sigma, float nu);
@@ -115,6 +138,11 @@ hipError_t target_order(const float4* pts, int n, int* perm, void* ws, size_t ws
 
 // Whether ordering the targets pays for a problem (a kernel with a
 // near-field-only path and enough targets to amortise the sort).
+// Sort sources and targets of a vorticity-field evaluation (tile skipping).
+inline bool vort_wants_sort(int n_src, int n_tgt) {
+  return n_src >= 4096 && n_tgt >= 1024 && (long long)n_src * n_tgt >= (1LL << 26);
+}
+
 inline bool p3d_wants_order(int reg, int n_src, int n_tgt) {
   return reg == REG_GAUSSIAN && n_tgt >= 4096 && (long long)n_src * n_tgt >= (1LL << 26);
 }

@@ -119,7 +119,9 @@ void* device_buffer(Runtime& rt, Device& d, int slot, size_t bytes) {
   if (bytes == 0) bytes = 16;
   if (b.bytes >= bytes) return b.p;
   if (b.p) {
-    (void)hipStreamSynchronize(d.stream);
+    // queued work on any stream (device-resident calls take the caller's
+    // stream) may still use the old buffer
+    (void)hipDeviceSynchronize();
     (void)hipFree(b.p);
     b = Buffer{};
   }

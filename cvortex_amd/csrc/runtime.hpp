@@ -30,6 +30,7 @@ enum Slot : int {
   SLOT_WORK,
   SLOT_ORDER,  // target-order workspace
   SLOT_PERM,   // target permutation
+  SLOT_PERM2,  // source permutation (vorticity field)
   SLOT_COUNT
 };
 

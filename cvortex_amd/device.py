@@ -32,6 +32,11 @@ class F3DProblem(ctypes.Structure):
                 ("n_tgt", _I), ("vel", _VP), ("dvort", _VP), ("accumulate", _I)]
 
 
+class VortProblem(ctypes.Structure):
+    _fields_ = [("src_pv", _VP), ("src_a", _VP), ("n_src", _I), ("tgt", _VP), ("n_tgt", _I),
+                ("vort", _VP), ("regularisation", _I), ("sigma", _F), ("accumulate", _I)]
+
+
 def p3d_soa(p):
     """cvtx_P3D records -> (pv, a) float32 (n, 4) arrays."""
     n = len(p)
@@ -81,6 +86,19 @@ def p3d_eval(prob, stream=None, shape=None):
         st = L.cvtx_hip_p3d_eval_shape(ctypes.byref(prob), shape[0], shape[1], stream)
     if st != 0:
         raise CvtxError(f"p3d_eval: {last_error()}")
+
+
+def vort_eval(prob, stream=None):
+    if lib().cvtx_hip_vort_eval(ctypes.byref(prob), stream) != 0:
+        raise CvtxError(f"vort_eval: {last_error()}")
+
+
+def relax_eval(pv, a, n, fdt, kernel, sigma, accumulate=1, stream=None):
+    """In-place Pedrizzetti relaxation of device strengths `a` (float4 rows)."""
+    reg = REG_ID[kernel] if isinstance(kernel, str) else int(kernel)
+    if lib().cvtx_hip_relax_eval(pv.addr, a.addr, n, float(fdt), reg, float(sigma), accumulate,
+                                 stream) != 0:
+        raise CvtxError(f"relax_eval: {last_error()}")
 
 
 def p2d_eval(prob, stream=None, shape=None):

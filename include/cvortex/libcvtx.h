@@ -3,8 +3,9 @@
  *
  * This header declares the subset of the reference cvortex C API that this
  * library implements: the element types, the regularisation plugin struct,
- * the accelerator controls and the seven many-to-many (M2M) direct-sum
- * Biot-Savart entry points.  Types, struct layouts and function signatures
+ * the accelerator controls, the seven many-to-many (M2M) direct-sum
+ * Biot-Savart entry points, and the vorticity field and relaxation built on
+ * the same pairwise machinery.  Types, struct layouts and function signatures
  * are ABI-identical to the reference header include/cvortex/libcvtx.h, so a
  * program (or the CVortex.jl wrapper) built against the reference header can
  * link against libcvortex.so instead.
@@ -129,6 +130,29 @@ CVTX_EXPORT void cvtx_P3D_M2M_visc_dvort(
 	const cvtx_VortFunc *kernel,
 	float regularisation_radius,
 	float kinematic_visc);
+
+/* 3D vortex particles: vorticity field at points.  Reference
+ * libcvtx.h:241-248, src/P3D.cpp:476-498 with the CPU semantics of
+ * src/P3D.cpp:298-322: only sources inside the box |dx|, |dy|, |dz| <
+ * 5 sigma of a point contribute, the self-pair included. */
+CVTX_EXPORT void cvtx_P3D_M2M_vort(
+	const cvtx_P3D **array_start,
+	const int num_particles,
+	const bsv_V3f *mes_start,
+	const int num_mes,
+	bsv_V3f *result_array,
+	const cvtx_VortFunc *kernel,
+	float regularisation_radius);
+
+/* 3D vortex particles: Pedrizzetti relaxation of the particle strengths
+ * towards the local vorticity field, in place.  Reference libcvtx.h:259-264,
+ * src/P3D.cpp:669-707. */
+CVTX_EXPORT void cvtx_P3D_pedrizzetti_relaxation(
+	cvtx_P3D **input_array_start,
+	const int n_input_particles,
+	float fdt,
+	const cvtx_VortFunc *kernel,
+	float regularisation_radius);
 
 /* Straight vortex filaments: velocity induced at points.  Reference
  * libcvtx.h:285-290, src/F3D.cpp:162-180. */

@@ -83,6 +83,19 @@ int cvtx_hip_P2D_M2M_visc_dvort(const cvtx_P2D** array_start, int num_particles,
                                 float* result_array, const cvtx_VortFunc* kernel,
                                 float regularisation_radius, float kinematic_visc);
 
+/* replaces opencl_brute_force_P3D_M2M_vort, reference src/ocl_P3D.cpp:121-147,728-899,
+ * with the reference CPU semantics (src/P3D.cpp:298-322): 5 sigma box
+ * cut-off, self-pair included.  (The reference's OpenCL kernel,
+ * src/ocl_kernels/nbody.cl:164-197, has no cut-off and drops the self-pair.) */
+int cvtx_hip_P3D_M2M_vort(const cvtx_P3D** array_start, int num_particles,
+                          const bsv_V3f* mes_start, int num_mes, bsv_V3f* result_array,
+                          const cvtx_VortFunc* kernel, float regularisation_radius);
+/* the whole of cvtx_P3D_pedrizzetti_relaxation (reference src/P3D.cpp:669-707)
+ * on the device: vorticity at the particles, then the strength update */
+int cvtx_hip_P3D_pedrizzetti_relaxation(cvtx_P3D** input_array_start, int n_input_particles,
+                                        float fdt, const cvtx_VortFunc* kernel,
+                                        float regularisation_radius);
+
 /* ---- 2. fused host-array entry points (no reference counterpart) ----
  * Any of the result pointers may be NULL (that output is not computed). */
 int cvtx_hip_P3D_M2M_fused(const cvtx_P3D** array_start, int num_particles,
@@ -164,12 +177,32 @@ typedef struct {
 	int accumulate;
 } cvtx_hip_f3d_problem;
 
+/* Vorticity field (cvtx_P3D_M2M_vort semantics): sources src_pv/src_a as
+ * above, targets tgt (float4 {x, y, z, -}), vort float[3*n_tgt]. */
+typedef struct {
+	const float* src_pv;
+	const float* src_a;
+	int n_src;
+	const float* tgt;
+	int n_tgt;
+	float* vort;
+	int regularisation; /* CVTX_HIP_REG_* */
+	float sigma;
+	int accumulate;     /* CVTX_HIP_ACC_* */
+} cvtx_hip_vort_problem;
+
 /* Evaluate with the tuned (or default) launch shape.  A source-partitioned
  * launch needs workspace; the library keeps a per-device workspace that it
  * grows on demand (this may allocate device memory on first use of a size). */
 int cvtx_hip_p3d_eval(const cvtx_hip_p3d_problem* p, void* stream);
 int cvtx_hip_p2d_eval(const cvtx_hip_p2d_problem* p, void* stream);
 int cvtx_hip_f3d_eval(const cvtx_hip_f3d_problem* p, void* stream);
+
+int cvtx_hip_vort_eval(const cvtx_hip_vort_problem* p, void* stream);
+/* Pedrizzetti relaxation of n device-resident particles (pv, a as above):
+ * a is updated in place (its 4th component is kept). */
+int cvtx_hip_relax_eval(const float* pv, float* a, int n, float fdt, int regularisation,
+                        float sigma, int accumulate, void* stream);
 
 /* Same, with an explicit launch shape: targets_per_lane in {1, 2, 4},
  * n_split >= 1 source partitions (rounded to whole 256-source tiles). */

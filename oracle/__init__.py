@@ -39,6 +39,12 @@ _SIGS = {
     "orc_P3D_M2M_vel": (None, [_VP, _I, _VP, _I, _VP, _I, _F]),
     "orc_P3D_M2M_dvort": (None, [_VP, _I, _VP, _I, _VP, _I, _F]),
     "orc_P3D_M2M_visc_dvort": (None, [_VP, _I, _VP, _I, _VP, _I, _F, _F]),
+    "orc_P3D_S2S_vort": (V3f, [ctypes.POINTER(P3D), V3f, _I, _F]),
+    "orc_P3D_M2S_vort": (V3f, [_VP, _I, V3f, _I, _F]),
+    "orc_P3D_M2M_vort": (None, [_VP, _I, _VP, _I, _VP, _I, _F]),
+    "orc_P3D_pedrizzetti_relaxation": (None, [_VP, _I, _F, _I, _F]),
+    "orc64_P3D_M2M_vort": (None, [_VP, _I, _VP, _VP, _I, _I, _D, _I, _VP]),
+    "orc64_P3D_pedrizzetti": (None, [_VP, _I, _D, _I, _D, _I, _VP]),
     "orc_F3D_S2S_vel": (V3f, [ctypes.POINTER(F3D), V3f]),
     "orc_F3D_S2S_dvort": (V3f, [ctypes.POINTER(F3D), ctypes.POINTER(P3D)]),
     "orc_F3D_M2M_vel": (None, [_VP, _I, _VP, _I, _VP]),
@@ -174,6 +180,28 @@ class Oracle:
                                       float(sigma), float(nu))
         return out
 
+    def P3D_S2S_vort(self, p, mes, kernel, sigma):
+        pp = P3D.from_buffer_copy(np.ascontiguousarray(p).tobytes())
+        m = V3f((ctypes.c_float * 3)(*mes))
+        r = self.L.orc_P3D_S2S_vort(ctypes.byref(pp), m, _reg(kernel), float(sigma))
+        return np.array(r.x[:], np.float32)
+
+    def P3D_M2M_vort(self, particles, mes, kernel, sigma):
+        p, keep = pointer_array(as_records(particles, P3D_DTYPE))
+        mes = np.ascontiguousarray(mes, np.float32).reshape(-1, 3)
+        out = np.empty_like(mes)
+        self.L.orc_P3D_M2M_vort(_vp(p), len(p), _vp(mes), len(mes), _vp(out), _reg(kernel),
+                                float(sigma))
+        return out
+
+    def P3D_pedrizzetti_relaxation(self, particles, fdt, kernel, sigma):
+        """Relaxed copy of the particle records (the input is not modified)."""
+        recs = np.array(as_records(particles, P3D_DTYPE), copy=True)
+        p, keep = pointer_array(recs)
+        self.L.orc_P3D_pedrizzetti_relaxation(_vp(p), len(recs), float(fdt), _reg(kernel),
+                                              float(sigma))
+        return recs
+
     def F3D_M2M_vel(self, fils, mes):
         f, keep = pointer_array(as_records(fils, F3D_DTYPE))
         mes = np.ascontiguousarray(mes, np.float32).reshape(-1, 3)
@@ -223,6 +251,24 @@ class Oracle:
         out = np.zeros((len(idx), 3))
         self.L.orc64_P3D_M2M_vel_pts(_vp(p), len(p), _vp(mes), _vp(idx), len(idx), _reg(kernel),
                                      float(sigma), gauss_mode, _vp(out))
+        return out
+
+    def f64_P3D_vort(self, particles, mes, kernel, sigma, idx=None, gauss_mode=1):
+        p, k1 = pointer_array(as_records(particles, P3D_DTYPE))
+        mes = np.ascontiguousarray(mes, np.float32).reshape(-1, 3)
+        idx = np.arange(len(mes), dtype=np.int32) if idx is None else np.ascontiguousarray(idx, np.int32)
+        out = np.zeros((len(idx), 3))
+        self.L.orc64_P3D_M2M_vort(_vp(p), len(p), _vp(mes), _vp(idx), len(idx), _reg(kernel),
+                                  float(sigma), gauss_mode, _vp(out))
+        return out
+
+    def f64_pedrizzetti(self, particles, fdt, kernel, sigma, gauss_mode=1):
+        """Relaxed strengths (n, 3) in double."""
+        recs = as_records(particles, P3D_DTYPE)
+        p, k1 = pointer_array(recs)
+        out = np.zeros((len(recs), 3))
+        self.L.orc64_P3D_pedrizzetti(_vp(p), len(recs), float(fdt), _reg(kernel), float(sigma),
+                                     gauss_mode, _vp(out))
         return out
 
     def f64_F3D(self, fils, induced, idx=None, vel=True, dvort=True):

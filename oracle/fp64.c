@@ -338,3 +338,71 @@ void orc64_P2D_M2M_vel_pts(const cvtx_P2D** src, int ns, const bsv_V2f* mes,
 		p2d_one(src, ns, mes[tgt_idx[k]].x, 0.f, 0.f, reg, sigma, 0.0, gauss_mode,
 		        vel + 2 * k, NULL);
 }
+
+/* ---------------- P3D vorticity field and relaxation ----------------
+ * reference src/P3D.cpp:298-322 (M2S_vort) and :669-707 (relaxation).  The
+ * 5 sigma box test is made in FP32 exactly as the reference makes it (the
+ * set of contributing sources is part of the definition); zeta, the sum and
+ * the normalisation are in double. */
+static double zeta3(int reg, double rho, int gmode) {
+	double G, Hn, z, g;
+	if (reg == ORC_SINGULAR) return 0.0;
+	if (reg == ORC_PLANETARY) return rho < 1.0 ? 3.0 : 0.0;
+	kernels3(reg, rho > 0.0 ? rho : 1e-300, gmode, &G, &Hn, &z, &g);
+	return z;
+}
+
+static void vort64_one(const cvtx_P3D** src, int ns, bsv_V3f mes, int reg, double sigma,
+                       int gmode, double* out) {
+	const float cutoff = 5.f * (float)sigma;
+	double sx = 0.0, sy = 0.0, sz = 0.0;
+	for (int i = 0; i < ns; ++i) {
+		const float dx = src[i]->coord.x[0] - mes.x[0];
+		const float dy = src[i]->coord.x[1] - mes.x[1];
+		const float dz = src[i]->coord.x[2] - mes.x[2];
+		if (!(fabsf(dx) < cutoff && fabsf(dy) < cutoff && fabsf(dz) < cutoff)) continue;
+		const double ex = (double)src[i]->coord.x[0] - mes.x[0];
+		const double ey = (double)src[i]->coord.x[1] - mes.x[1];
+		const double ez = (double)src[i]->coord.x[2] - mes.x[2];
+		const double z = zeta3(reg, sqrt(ex * ex + ey * ey + ez * ez) / sigma, gmode);
+		sx += z * src[i]->vorticity.x[0];
+		sy += z * src[i]->vorticity.x[1];
+		sz += z * src[i]->vorticity.x[2];
+	}
+	const double k = 1.0 / (4.0 * PI_D * sigma * sigma * sigma);
+	out[0] = sx * k;
+	out[1] = sy * k;
+	out[2] = sz * k;
+}
+
+void orc64_P3D_M2M_vort(const cvtx_P3D** src, int ns, const bsv_V3f* mes,
+                        const int* tgt_idx, int n_idx, int reg, double sigma,
+                        int gauss_mode, double* out) {
+	int k;
+#pragma omp parallel for schedule(dynamic, 16)
+	for (k = 0; k < n_idx; ++k)
+		vort64_one(src, ns, mes[tgt_idx[k]], reg, sigma, gauss_mode, out + 3 * (size_t)k);
+}
+
+void orc64_P3D_pedrizzetti(const cvtx_P3D** p, int n, double fdt, int reg, double sigma,
+                           int gauss_mode, double* alpha_out) {
+	int i;
+#pragma omp parallel for schedule(dynamic, 16)
+	for (i = 0; i < n; ++i) {
+		double w[3];
+		vort64_one(p, n, p[i]->coord, reg, sigma, gauss_mode, w);
+		const double ax = p[i]->vorticity.x[0], ay = p[i]->vorticity.x[1],
+		             az = p[i]->vorticity.x[2];
+		const double aw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+		const double aa = sqrt(ax * ax + ay * ay + az * az);
+		double* o = alpha_out + 3 * (size_t)i;
+		if (aw == 0.0) {
+			o[0] = o[1] = o[2] = 0.0;
+		} else {
+			const double c = aa / aw * fdt;
+			o[0] = (1.0 - fdt) * ax + c * w[0];
+			o[1] = (1.0 - fdt) * ay + c * w[1];
+			o[2] = (1.0 - fdt) * az + c * w[2];
+		}
+	}
+}

@@ -52,6 +52,12 @@ void orc_P3D_M2M_dvort(const cvtx_P3D** src, int ns, const cvtx_P3D** ind, int n
 void orc_P3D_M2M_visc_dvort(const cvtx_P3D** src, int ns, const cvtx_P3D** ind, int ni,
                             bsv_V3f* out, int reg, float sigma, float nu);
 
+bsv_V3f orc_P3D_S2S_vort(const cvtx_P3D* self, bsv_V3f mes, int reg, float sigma);
+bsv_V3f orc_P3D_M2S_vort(const cvtx_P3D** src, int ns, bsv_V3f mes, int reg, float sigma);
+void orc_P3D_M2M_vort(const cvtx_P3D** src, int ns, const bsv_V3f* mes, int nm,
+                      bsv_V3f* out, int reg, float sigma);
+void orc_P3D_pedrizzetti_relaxation(cvtx_P3D** p, int n, float fdt, int reg, float sigma);
+
 bsv_V3f orc_F3D_S2S_vel(const cvtx_F3D* self, bsv_V3f mes);
 bsv_V3f orc_F3D_S2S_dvort(const cvtx_F3D* self, const cvtx_P3D* induced);
 void orc_F3D_M2M_vel(const cvtx_F3D** src, int nf, const bsv_V3f* mes, int nm, bsv_V3f* out);
@@ -77,6 +83,14 @@ void orc64_P3D_M2M(const cvtx_P3D** src, int ns, const cvtx_P3D** ind,
 void orc64_P3D_M2M_vel_pts(const cvtx_P3D** src, int ns, const bsv_V3f* mes,
                            const int* tgt_idx, int n_idx, int reg, double sigma,
                            int gauss_mode, double* vel);
+/* Vorticity field with the reference's 5 sigma box cut-off (the box test
+ * is made in FP32 exactly as the reference makes it, the rest in double),
+ * and the Pedrizzetti relaxation of n particles: alpha_out[3 n] (double). */
+void orc64_P3D_M2M_vort(const cvtx_P3D** src, int ns, const bsv_V3f* mes,
+                        const int* tgt_idx, int n_idx, int reg, double sigma,
+                        int gauss_mode, double* out);
+void orc64_P3D_pedrizzetti(const cvtx_P3D** p, int n, double fdt, int reg, double sigma,
+                           int gauss_mode, double* alpha_out);
 void orc64_F3D_M2M(const cvtx_F3D** src, int nf, const cvtx_P3D** ind,
                    const int* tgt_idx, int n_idx, double* vel, double* dvort);
 void orc64_F3D_M2M_vel_pts(const cvtx_F3D** src, int nf, const bsv_V3f* mes,

@@ -23,6 +23,7 @@
  */
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
@@ -320,6 +321,82 @@ void orc_P3D_M2M_visc_dvort(const cvtx_P3D** src, int ns, const cvtx_P3D** ind, 
 		}
 		out[i] = v3((float)sx, (float)sy, (float)sz);
 	}
+}
+
+/* ------------------------------------------------------------------ */
+/* P3D vorticity field and Pedrizzetti relaxation: reference           */
+/* src/P3D.cpp:146-161, 213-228, 298-322, 458-498, 669-707             */
+/* ------------------------------------------------------------------ */
+
+/* reference src/P3D.cpp:146-161: no cut-off, rho = |r| / sigma (signed) */
+bsv_V3f orc_P3D_S2S_vort(const cvtx_P3D* self, bsv_V3f mes, int reg, float sigma) {
+	orc_vortfunc f = orc_vf(reg);
+	bsv_V3f d = v3_sub(self->coord, mes);
+	float r = v3_norm(d);
+	float coeff = f.zeta3(r / sigma);
+	float divisor = 4.f * REF_PI_F * sigma * sigma * sigma;
+	coeff = coeff / divisor;
+	return v3_scale(self->vorticity, coeff);
+}
+
+/* reference src/P3D.cpp:298-322: sources outside the box |dx|,|dy|,|dz| <
+ * 5 sigma are skipped; the sum is a running FP32 vector sum; the
+ * self-pair is included (zeta(0) is finite). */
+static bsv_V3f p3d_m2s_vort(const cvtx_P3D** src, int ns, bsv_V3f mes,
+                            const orc_vortfunc* f, float sigma) {
+	float cutoff = (ORC_MUT(13) ? 4.f : 5.f) * sigma;
+	float rsigma = 1 / sigma;
+	bsv_V3f sum = v3(0.f, 0.f, 0.f);
+	for (long i = 0; i < ns; ++i) {
+		bsv_V3f d = v3_sub(src[i]->coord, mes);
+		int in = ORC_MUT(14) ? (v3_norm(d) < cutoff)
+		                     : (fabsf(d.x[0]) < cutoff && fabsf(d.x[1]) < cutoff &&
+		                        fabsf(d.x[2]) < cutoff);
+		if (in) {
+			float r = v3_norm(d);
+			float coeff = f->zeta3(r * rsigma);
+			sum = v3_add(v3_scale(src[i]->vorticity, coeff), sum);
+		}
+	}
+	return v3_divs(sum, 4.f * REF_PI_F * sigma * sigma * sigma);
+}
+
+bsv_V3f orc_P3D_M2S_vort(const cvtx_P3D** src, int ns, bsv_V3f mes, int reg, float sigma) {
+	orc_vortfunc f = orc_vf(reg);
+	return p3d_m2s_vort(src, ns, mes, &f, sigma);
+}
+
+/* reference src/P3D.cpp:458-474 */
+void orc_P3D_M2M_vort(const cvtx_P3D** src, int ns, const bsv_V3f* mes, int nm,
+                      bsv_V3f* out, int reg, float sigma) {
+	orc_vortfunc f = orc_vf(reg);
+	long i;
+#pragma omp parallel for schedule(guided)
+	for (i = 0; i < nm; ++i)
+		out[i] = p3d_m2s_vort(src, ns, mes[i], &f, sigma);
+}
+
+/* reference src/P3D.cpp:669-707: omega = M2M_vort at the particles, then
+ * alpha <- (1 - fdt) alpha + fdt omega |alpha| / |omega|, and alpha <- 0
+ * where |omega| == 0.  Updates the records in place. */
+void orc_P3D_pedrizzetti_relaxation(cvtx_P3D** p, int n, float fdt, int reg, float sigma) {
+	bsv_V3f* mes = (bsv_V3f*)malloc(sizeof(bsv_V3f) * (size_t)(n > 0 ? n : 1));
+	bsv_V3f* om = (bsv_V3f*)malloc(sizeof(bsv_V3f) * (size_t)(n > 0 ? n : 1));
+	long i;
+	for (i = 0; i < n; ++i) mes[i] = p[i]->coord;
+	orc_P3D_M2M_vort((const cvtx_P3D**)p, n, mes, n, om, reg, sigma);
+	float tmp = 1.f - fdt;
+#pragma omp parallel for
+	for (i = 0; i < n; ++i) {
+		bsv_V3f ov = p[i]->vorticity;
+		float absomega = v3_norm(om[i]);
+		float coeff = v3_norm(ov) / absomega;
+		bsv_V3f nv = v3_scale(ov, ORC_MUT(15) ? fdt : tmp);
+		nv = v3_add(nv, v3_scale(om[i], coeff * fdt));
+		p[i]->vorticity = absomega != 0.f ? nv : v3(0.f, 0.f, 0.f);
+	}
+	free(mes);
+	free(om);
 }
 
 /* ------------------------------------------------------------------ */

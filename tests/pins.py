@@ -427,6 +427,10 @@ def refcpu_vs_fp64(o):
         f.append("F3D vel FP32 vs FP64")
     if orc.rel_l2(o.F3D_M2M_dvort(fl, t), d64) > 5e-6:
         f.append("F3D dvort FP32 vs FP64")
+    for k in REGS:
+        g64 = o.f64_P3D_vort(p, t["coord"], k, 0.3, gauss_mode=1)
+        if np.linalg.norm(g64) > 0 and orc.rel_l2(o.P3D_M2M_vort(p, t["coord"], k, 0.3), g64) > 5e-6:
+            f.append(f"{k}: P3D vort FP32 vs FP64")
     q = W.random_p2d(160, 24, box=1.0, vort=1.0, area=0.01)
     r = W.random_p2d(96, 25, box=1.0, vort=1.0, area=0.01)
     for k in REGS:
@@ -439,6 +443,103 @@ def refcpu_vs_fp64(o):
     return f
 
 
+# ---------------------------------------------------------------------------
+# H. vorticity field: reference src/P3D.cpp:146-161 (S2S, no cut-off) and
+#    :298-322 (M2S/M2M: sources outside the box |dx|, |dy|, |dz| < 5 sigma
+#    are skipped); normalisation checked by quadrature of the field
+# ---------------------------------------------------------------------------
+def _zeta_textbook(k, rho):
+    # Winckelmans high-order algebraic and Gaussian vorticity distributions;
+    # planetary: uniform core of unit volume in rho; singular: none
+    if k == "winckelmans":
+        return 7.5 * (rho * rho + 1.0) ** -3.5
+    if k == "gaussian":
+        return math.sqrt(2.0 / math.pi) * math.exp(-0.5 * rho * rho)
+    if k == "planetary":
+        return 3.0 if rho < 1.0 else 0.0
+    return 0.0
+
+
+def vort_box_cutoff(o):
+    f = []
+    s = 0.2
+    alpha = np.array([0.3, -0.5, 0.7])
+    offsets = [((0.0, 0.0, 0.0), True), ((0.9, 0.0, 0.0), True), ((3.0, 0.0, 0.0), True),
+               ((4.5, 0.0, 0.0), True), ((0.0, -4.9, 0.0), True), ((4.0, 4.0, 4.0), True),
+               ((5.1, 0.0, 0.0), False), ((0.0, 0.0, -5.2), False), ((3.0, 5.3, 1.0), False)]
+    x0 = np.array([0.1, -0.2, 0.3])
+    for k in REGS:
+        for off, inside in offsets:
+            src = make_p3d([x0 + s * np.array(off)], [alpha], [0.01])
+            got = o.P3D_M2M_vort(src, x0.reshape(1, 3), k, s)[0].astype(np.float64)
+            rho = math.sqrt(sum(c * c for c in off))
+            want = alpha * _zeta_textbook(k, rho) / (4 * math.pi * s ** 3) if inside else 0 * alpha
+            if inside and np.any(want != 0):
+                if np.linalg.norm(got - want) > 2e-5 * np.linalg.norm(want):
+                    f.append(f"{k}: M2M_vort single source at {off} sigma")
+            elif np.any(got != 0):
+                f.append(f"{k}: M2M_vort source at {off} sigma must not contribute")
+        # S2S has no cut-off (reference src/P3D.cpp:146-161)
+        src = make_p3d([x0 + s * np.array([5.5, 0.0, 0.0])], [alpha], [0.01])
+        got = o.P3D_S2S_vort(src[0], x0, k, s).astype(np.float64)
+        want = alpha * _zeta_textbook(k, 5.5) / (4 * math.pi * s ** 3)
+        if np.linalg.norm(got - want) > 2e-5 * max(np.linalg.norm(want), 1e-30):
+            f.append(f"{k}: S2S_vort has no cut-off")
+        # negative radius: the cut-off 5 sigma is negative, nothing contributes
+        src = make_p3d([x0], [alpha], [0.01])
+        if np.any(o.P3D_M2M_vort(src, x0.reshape(1, 3), k, -s) != 0):
+            f.append(f"{k}: negative sigma gives zero vorticity")
+    # normalisation: the Gaussian field of one particle integrates to alpha
+    h = s / 4
+    ax = (np.arange(-20, 20) + 0.5) * h
+    X, Y, Z = np.meshgrid(ax, ax, ax, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.float32)
+    src = make_p3d([[0.0, 0.0, 0.0]], [alpha], [0.01])
+    w = o.P3D_M2M_vort(src, pts, "gaussian", s).astype(np.float64)
+    tot = w.sum(axis=0) * h ** 3
+    if np.linalg.norm(tot - alpha) > 1e-3 * np.linalg.norm(alpha):
+        f.append(f"gaussian vorticity field integrates to {tot}, not alpha")
+    return f
+
+
+# ---------------------------------------------------------------------------
+# I. Pedrizzetti relaxation, reference src/P3D.cpp:669-707:
+#    alpha <- (1 - fdt) alpha + fdt |alpha| omega/|omega|; alpha <- 0 where
+#    omega = 0.  Checked through invariants and the FP64 oracle.
+# ---------------------------------------------------------------------------
+def relaxation_invariants(o):
+    f = []
+    s = 0.1
+    # isolated particles: omega is parallel to alpha, so nothing changes
+    iso = make_p3d([[0, 0, 0], [1, 0, 0], [0, 2, 0]], [[0.3, -0.2, 0.5], [1, 1, 0], [0, 0, -2]],
+                   [0.01] * 3)
+    for k in ("winckelmans", "gaussian", "planetary"):
+        for fdt in (0.0, 0.3, 1.0):
+            r = o.P3D_pedrizzetti_relaxation(iso, fdt, k, s)
+            if np.abs(r["vorticity"] - iso["vorticity"]).max() > 1e-6:
+                f.append(f"{k}: isolated particles change under relaxation (fdt={fdt})")
+    # no vorticity field (singular: zeta = 0): every strength is zeroed
+    r = o.P3D_pedrizzetti_relaxation(iso, 0.3, "singular", s)
+    if np.any(r["vorticity"] != 0):
+        f.append("singular: relaxation with omega = 0 must zero the strengths")
+    cl = W.random_p3d(300, 26, box=0.5, vort=1.0, vol=0.001)
+    om = o.P3D_M2M_vort(cl, cl["coord"], "winckelmans", s).astype(np.float64)
+    r0 = o.P3D_pedrizzetti_relaxation(cl, 0.0, "winckelmans", s)
+    if np.abs(r0["vorticity"] - cl["vorticity"]).max() > 1e-6:
+        f.append("fdt = 0 must leave the strengths unchanged")
+    r1 = o.P3D_pedrizzetti_relaxation(cl, 1.0, "winckelmans", s)["vorticity"].astype(np.float64)
+    a = cl["vorticity"].astype(np.float64)
+    want1 = om / np.linalg.norm(om, axis=1, keepdims=True) * np.linalg.norm(a, axis=1, keepdims=True)
+    if np.abs(r1 - want1).max() > 1e-5 * np.abs(want1).max():
+        f.append("fdt = 1 must align alpha with omega at fixed |alpha|")
+    for k in ("winckelmans", "gaussian"):
+        got = o.P3D_pedrizzetti_relaxation(cl, 0.25, k, s)["vorticity"]
+        if orc.rel_l2(got, o.f64_pedrizzetti(cl, 0.25, k, s)) > 5e-6:
+            f.append(f"{k}: relaxation FP32 vs FP64")
+    return f
+
+
 ALL_CHECKS = (kat_vortfunc, moments_3d, g_cumulative_zeta, kernels_2d, particle_invariants,
               biot_savart_pair, dvort_transpose_fd, pse_conservation, f3d_vel_quadrature,
-              f3d_dvort_structure, p2d_point_vortex, ring_centre, refcpu_vs_fp64)
+              f3d_dvort_structure, p2d_point_vortex, ring_centre, refcpu_vs_fp64,
+              vort_box_cutoff, relaxation_invariants)

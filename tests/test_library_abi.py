@@ -160,6 +160,14 @@ def test_no_device_public_api_reports_nan(L):
         assert np.isnan(out).all()
         with pytest.raises(cv.CvtxError):
             cv.P3D_M2M_vel(p, p["coord"], "singular", 0.1)
+        out[:] = 0
+        L.cvtx_P3D_M2M_vort(ctypes.c_void_p(ptrs.ctypes.data), 3, ctypes.c_void_p(mes.ctypes.data), 3,
+                            ctypes.c_void_p(out.ctypes.data), ctypes.byref(vf), 0.1)
+        assert np.isnan(out).all()
+        # the relaxation's result is the strengths themselves
+        L.cvtx_P3D_pedrizzetti_relaxation(ctypes.c_void_p(ptrs.ctypes.data), 3, 0.1,
+                                          ctypes.byref(vf), 0.1)
+        assert np.isnan(p["vorticity"]).all()
     finally:
         for i, st in enumerate(states):
             (cv.accelerator_enable if st else cv.accelerator_disable)(i)

@@ -30,6 +30,9 @@ EXPECTED = {
     10: "ring_centre",          # M2M loop drops the last source
     11: "kernels_2d",           # Gaussian eta_2D exp(-rho^2/2) -> exp(-rho^2)
     12: "kernels_2d",           # Winckelmans g_2D numerator rho^4 + 2 rho^2 -> rho^4 + rho^2
+    13: "vort_box_cutoff",      # vorticity cut-off 5 sigma -> 4 sigma
+    14: "vort_box_cutoff",      # vorticity cut-off box -> sphere
+    15: "relaxation_invariants",  # relaxation keeps fdt alpha instead of (1 - fdt) alpha
 }
 
 
