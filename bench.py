@@ -50,6 +50,9 @@ FLOPS_PER_PAIR = {
     ("f3d", "vel", "singular"): 56, ("f3d", "dvort", "singular"): 76,
     ("f3d", "vel+dvort", "singular"): 100,
     ("p2d", "vel", "planetary"): 14,
+    # vorticity field, counted with the same convention: d 3, box test 5
+    # (3 compares, 2 ands), r^2 5, rho^2 1, zeta (exp 2), select 1, 3 FMA 6
+    ("vort", "vort", "gaussian"): 23,
 }
 METRIC = "P3D M2M vel+dvort interactions/s at N=1M; % FP32 VALU peak; rel err vs FP64"
 
@@ -447,8 +450,87 @@ def bench_sheet(args, cv, D, W, o, stream):
     return line
 
 
+# --------------------------------------------------------------------------
+# Vorticity field + Pedrizzetti relaxation on the ring, N = 2^20
+# --------------------------------------------------------------------------
+def bench_relax(args, cv, D, W, o, stream):
+    n = args.n or (1 << 20)
+    p, sigma, nu = W.ring(n)
+    fdt = 0.1
+    pv, a = D.p3d_soa(p)
+    dpv, da = D.DeviceArray.from_host(pv), D.DeviceArray.from_host(a)
+    om = D.DeviceArray((n, 3))
+    tm = Timer(stream)
+    reg = 3  # gaussian
+    prob = D.VortProblem(dpv.addr, da.addr, n, dpv.addr, n, om.addr, reg, sigma, cv.ACC_COMP)
+    log(f"relax ring n={n}: vorticity field (sorted, tile skipping)")
+    ms_vort, k_vort = tm.run(lambda: D.vort_eval(prob, stream), args.steps, args.warmup)
+    W_gpu = om.download()
+    # one relaxation step on a copy of the strengths (the timed steps update
+    # the copy in place; the step count does not change the work)
+    da2 = D.DeviceArray.from_host(a)
+    ms_rel, k_rel = tm.run(lambda: D.relax_eval(dpv, da2, n, fdt, "gaussian", sigma, stream=stream),
+                           args.steps, 0)
+    da3 = D.DeviceArray.from_host(a)
+    D.relax_eval(dpv, da3, n, fdt, "gaussian", sigma, stream=stream)
+    D.synchronize()
+    A_gpu = da3.download()[:, :3]
+    sweep = {}
+    for T in (1, 2, 4):
+        _, kT = tm.run(lambda: D.vort_eval(prob, stream, shape=(T, 1)), 2, 1)
+        sweep[f"T={T}"] = kT
+    log(f"  sorted vort by targets per lane: {sweep}")
+    # brute force: the same kernel without the spatial sort (every tile evaluated)
+    L = cv.lib()
+    prev = L.cvtx_hip_set_target_order(0)
+    try:
+        om2 = D.DeviceArray((n, 3))
+        prob2 = D.VortProblem(dpv.addr, da.addr, n, dpv.addr, n, om2.addr, reg, sigma, cv.ACC_COMP)
+        ms_b, k_b = tm.run(lambda: D.vort_eval(prob2, stream), max(1, args.steps - 3), 1)
+    finally:
+        L.cvtx_hip_set_target_order(prev)
+    log(f"  vort {k_vort:.2f} ms (sorted) vs {k_b:.1f} ms (brute force); relax step {ms_rel:.2f} ms")
+    import oracle
+    same = oracle.rel_l2(om2.download(), W_gpu)
+    rng = np.random.default_rng(0x5EED + 6)
+    idx = np.sort(rng.choice(n, size=min(n_err(args, 2048) or 2048, n), replace=False)).astype(np.int32)
+    w64 = o.f64_P3D_vort(p, p["coord"], "gaussian", sigma, idx=idx)
+    a64 = p["vorticity"][idx].astype(np.float64)
+    aw = np.linalg.norm(w64, axis=1, keepdims=True)
+    aa = np.linalg.norm(a64, axis=1, keepdims=True)
+    r64 = np.where(aw > 0, (1 - fdt) * a64 + fdt * aa / np.where(aw > 0, aw, 1) * w64, 0.0)
+    err = {"samples": int(len(idx)), "vort": err_stats(W_gpu[idx], w64),
+           "relaxed_strength": err_stats(A_gpu[idx], r64)}
+    cpu = None
+    if not args.no_cpu:
+        m = min(args.cpu_samples, n)
+        tg = np.ascontiguousarray(p["coord"][idx[:m]])
+        best, rv = cpu_time(lambda: o.P3D_M2M_vort(p, tg, "gaussian", sigma))
+        cpu = {"value": m / best, "unit": "targets/s", "cores": int(o.L.orc_num_threads()),
+               "kind": "port",
+               "sample": f"cvtx_P3D_M2M_vort at {m} sampled ring particles x all {n} sources "
+                         f"(box cut-off test per pair), best of 2, {best:.2f} s",
+               "rel_l2_gpu_vs_cpu": oracle.rel_l2(W_gpu[idx[:m]], rv),
+               "rel_l2_cpu_vs_fp64": oracle.rel_l2(rv, w64[:m])}
+    pairs = float(n) * n
+    F = FLOPS_PER_PAIR[("vort", "vort", "gaussian")]
+    line = base_line(args, n / (ms_rel * 1e-3), "particles/s", ms_rel,
+                     {"workload": "Pedrizzetti relaxation step on the thin vortex ring (R=1, core "
+                                  "0.05), Gaussian sigma=0.02, fdt=0.1: vorticity field with the "
+                                  "5 sigma box cut-off, then the strength update",
+                      "n_particles": n})
+    line.update({"roofline": dict(roofline(pairs, F, k_b), note="brute-force kernel (no sort): "
+                                  "all 2^40 pairs evaluated"),
+                 "vort_sorted_ms": k_vort, "vort_brute_force_ms": k_b,
+                 "vort_sorted_ms_by_targets_per_lane": sweep,
+                 "speedup_of_tile_skipping": k_b / k_vort,
+                 "rel_l2_sorted_vs_brute_force": same,
+                 "rel_err_vs_fp64": err, "cpu_baseline": cpu})
+    return line
+
+
 WORKLOADS = {"ring": bench_ring, "cube": bench_cube, "lattice": bench_lattice,
-             "clouds": bench_clouds, "sheet": bench_sheet}
+             "clouds": bench_clouds, "sheet": bench_sheet, "relax": bench_relax}
 
 
 def main():

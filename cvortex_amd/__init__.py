@@ -86,6 +86,7 @@ _SIGNATURES = {
     # device-resident
     "cvtx_hip_p3d_eval": (_I, [_VP, _VP]),
     "cvtx_hip_vort_eval": (_I, [_VP, _VP]),
+    "cvtx_hip_vort_eval_shape": (_I, [_VP, _I, _I, _VP]),
     "cvtx_hip_relax_eval": (_I, [_VP, _VP, _I, _F, _I, _F, _I, _VP]),
     "cvtx_hip_p2d_eval": (_I, [_VP, _VP]),
     "cvtx_hip_f3d_eval": (_I, [_VP, _VP]),

@@ -116,6 +116,27 @@ void report_failure(const char* fn, float* out, long n) {
 
 hipStream_t pick_stream(Device* d, void* s) { return s ? (hipStream_t)s : d->stream; }
 
+bool make_launch_shape(int T, int n_split, int n_src, LaunchCfg* c) {
+  if (T != 1 && T != 2 && T != 4) return false;
+  if (n_split < 1) return false;
+  c->T = T;
+  c->wg = 256;
+  const long tiles = ((long)n_src + 255) / 256;
+  if (n_split == 1 || tiles <= 1) {
+    c->n_split = 1;
+    c->split_len = 0;
+    return true;
+  }
+  const long per = (tiles + n_split - 1) / n_split;
+  c->split_len = (int)(per * 256);
+  c->n_split = (int)((n_src + c->split_len - 1) / c->split_len);
+  if (c->n_split <= 1) {
+    c->n_split = 1;
+    c->split_len = 0;
+  }
+  return true;
+}
+
 int* order_points(Runtime& rt, Device& d, const float4* pts, int n, int slot, hipStream_t s) {
   const size_t wb = target_order_workspace(n);
   void* ws = device_buffer(rt, d, SLOT_ORDER, wb);
@@ -716,26 +737,6 @@ static int f3d_dev(const cvtx_hip_f3d_problem* p, const LaunchCfg* forced, void*
   return check_hip(rt, launch_f3d(j, c, pick_stream(d, stream)), "F3D launch") ? 0 : -1;
 }
 
-static bool make_shape(int T, int n_split, int n_src, LaunchCfg* c) {
-  if (T != 1 && T != 2 && T != 4) return false;
-  if (n_split < 1) return false;
-  c->T = T;
-  c->wg = 256;
-  const long tiles = ((long)n_src + 255) / 256;
-  if (n_split == 1 || tiles <= 1) {
-    c->n_split = 1;
-    c->split_len = 0;
-    return true;
-  }
-  const long per = (tiles + n_split - 1) / n_split;
-  c->split_len = (int)(per * 256);
-  c->n_split = (int)((n_src + c->split_len - 1) / c->split_len);
-  if (c->n_split <= 1) {
-    c->n_split = 1;
-    c->split_len = 0;
-  }
-  return true;
-}
 
 int cvtx_hip_p3d_eval(const cvtx_hip_p3d_problem* p, void* stream) { return p3d_dev(p, nullptr, stream); }
 int cvtx_hip_p2d_eval(const cvtx_hip_p2d_problem* p, void* stream) { return p2d_dev(p, nullptr, stream); }
@@ -743,7 +744,7 @@ int cvtx_hip_f3d_eval(const cvtx_hip_f3d_problem* p, void* stream) { return f3d_
 
 int cvtx_hip_p3d_eval_shape(const cvtx_hip_p3d_problem* p, int T, int n_split, void* stream) {
   LaunchCfg c;
-  if (!p || !make_shape(T, n_split, p->n_src, &c)) {
+  if (!p || !make_launch_shape(T, n_split, p->n_src, &c)) {
     set_error(runtime(), "invalid launch shape");
     return -1;
   }
@@ -752,7 +753,7 @@ int cvtx_hip_p3d_eval_shape(const cvtx_hip_p3d_problem* p, int T, int n_split, v
 
 int cvtx_hip_p2d_eval_shape(const cvtx_hip_p2d_problem* p, int T, int n_split, void* stream) {
   LaunchCfg c;
-  if (!p || !make_shape(T, n_split, p->n_src, &c)) {
+  if (!p || !make_launch_shape(T, n_split, p->n_src, &c)) {
     set_error(runtime(), "invalid launch shape");
     return -1;
   }
@@ -761,7 +762,7 @@ int cvtx_hip_p2d_eval_shape(const cvtx_hip_p2d_problem* p, int T, int n_split, v
 
 int cvtx_hip_f3d_eval_shape(const cvtx_hip_f3d_problem* p, int T, int n_split, void* stream) {
   LaunchCfg c;
-  if (!p || !make_shape(T, n_split, p->n_fil, &c)) {
+  if (!p || !make_launch_shape(T, n_split, p->n_fil, &c)) {
     set_error(runtime(), "invalid launch shape");
     return -1;
   }

@@ -8,6 +8,7 @@
 
 #include <cvortex/libcvtx.h>
 
+#include "kernels.hpp"
 #include "runtime.hpp"
 
 namespace cvtx {
@@ -31,5 +32,8 @@ void report_failure(const char* fn, float* out, long n);
 hipStream_t pick_stream(Device* d, void* s);
 // Morton permutation of n points into runtime slot `slot` (nullptr on failure)
 int* order_points(Runtime& rt, Device& d, const float4* pts, int n, int slot, hipStream_t s);
+// explicit launch shape: T in {1, 2, 4}, n_split >= 1 source partitions
+// (rounded to whole 256-source tiles); false for an invalid shape
+bool make_launch_shape(int T, int n_split, int n_src, LaunchCfg* c);
 
 }  // namespace cvtx

@@ -117,6 +117,8 @@ size_t vort_workspace_bytes(const VortJob& j, const LaunchCfg& c, bool sorted);
 hipError_t launch_vort(const VortJob& j, const LaunchCfg& c, void* ws, size_t ws_bytes,
                        hipStream_t s);
 double vort_scale(int reg, float sigma);
+// Default shape of a vorticity-field launch (sorted: with tile skipping).
+LaunchCfg vort_default_cfg(int n_src, int n_tgt, int n_cu, bool sorted);
 // Pedrizzetti update of a[n] (float4 {alpha, w}) from omega[3 n], in place.
 hipError_t launch_relax(float4* a, const float* omega, int n, float fdt, hipStream_t s);
 

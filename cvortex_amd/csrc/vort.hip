@@ -14,11 +14,12 @@
 // workgroup, T targets per lane, FP32 tile sums combined by TwoSum, optional
 // source partitions with an FP64 reduction.  Because the cut-off makes the
 // interaction short-ranged, large problems run on Morton-sorted sources and
-// targets: each 256-source tile carries its bounding box, each workgroup
-// computes the box of its targets, and a tile whose box is separated from
-// the workgroup's by at least the cut-off along any axis is skipped.  FP32
-// subtraction is monotonic, so every pair of a skipped tile would fail the
-// reference's own box test: the skip changes no result.
+// targets: each 256-source tile and each of its 64-source chunks carries its
+// bounding box; a workgroup skips a tile, and a wave a chunk, whose box is
+// separated from the box of its own targets by at least the cut-off along
+// any axis.  FP32 subtraction is monotonic, so every pair of a skipped tile
+// or chunk would fail the reference's own box test: skipping changes no
+// result.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -81,6 +82,8 @@ struct VortKParams {
   const int* __restrict__ tgt_perm;   // slot -> target (nullptr: identity)
   const float4* __restrict__ box_lo;  // per 256-source tile (nullptr: no skipping)
   const float4* __restrict__ box_hi;
+  const float4* __restrict__ sub_lo;  // per 64-source chunk
+  const float4* __restrict__ sub_hi;
   float* __restrict__ out;
   double* __restrict__ partial;
   int n_src, n_tgt, split_len, n_split;
@@ -115,11 +118,14 @@ __global__ __launch_bounds__(WG) void vort_kernel(VortKParams p) {
     const float4 q = p.tgt[i];
     xt[k] = make_float3(q.x, q.y, q.z);
   }
-  // bounding box of this workgroup's targets (used only with tile boxes)
+  // bounding boxes of this wave's and this workgroup's targets (used only
+  // with tile boxes)
   float3 wlo = make_float3(0.f, 0.f, 0.f), whi = wlo;
+  float3 lo = wlo, hi = wlo;
   const bool skipping = p.box_lo != nullptr;
   if (skipping) {
-    float3 lo = xt[0], hi = xt[0];
+    lo = xt[0];
+    hi = xt[0];
 #pragma unroll
     for (int k = 1; k < T; ++k) {
       lo.x = fminf(lo.x, xt[k].x); lo.y = fminf(lo.y, xt[k].y); lo.z = fminf(lo.z, xt[k].z);
@@ -143,11 +149,11 @@ __global__ __launch_bounds__(WG) void vort_kernel(VortKParams p) {
     }
   }
 
-  float acc[T][3], hi[T][3], lo[T][3];
+  float acc[T][3], sh[T][3], sl[T][3];
 #pragma unroll
   for (int k = 0; k < T; ++k)
 #pragma unroll
-    for (int a = 0; a < 3; ++a) { acc[k][a] = 0.f; hi[k][a] = 0.f; lo[k][a] = 0.f; }
+    for (int a = 0; a < 3; ++a) { acc[k][a] = 0.f; sh[k][a] = 0.f; sl[k][a] = 0.f; }
 
   const float c = p.cutoff;
   const int s0 = blockIdx.y * p.split_len;
@@ -174,29 +180,41 @@ __global__ __launch_bounds__(WG) void vort_kernel(VortKParams p) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) acc[k][a] = 0.f;
     }
+    // 64-source chunks; with boxes, a wave skips a chunk whose box is apart
+    // from the wave's own targets (uniform within the wave)
+    for (int j0 = 0; j0 < cnt; j0 += 64) {
+      if (skipping) {
+        const float4 bl = p.sub_lo[(ts + j0) / 64];
+        const float4 bh = p.sub_hi[(ts + j0) / 64];
+        const bool apart = (bl.x - hi.x >= c) || (lo.x - bh.x >= c) || (bl.y - hi.y >= c) ||
+                           (lo.y - bh.y >= c) || (bl.z - hi.z >= c) || (lo.z - bh.z >= c);
+        if (apart) continue;
+      }
+      const int j1 = min(cnt, j0 + 64);
 #pragma unroll 2
-    for (int j = 0; j < cnt; ++j) {
-      const float4 sp = s_pv[j];
-      const float4 sa = s_a[j];
+      for (int j = j0; j < j1; ++j) {
+        const float4 sp = s_pv[j];
+        const float4 sa = s_a[j];
 #pragma unroll
-      for (int k = 0; k < T; ++k) {
-        const float dx = sp.x - xt[k].x;
-        const float dy = sp.y - xt[k].y;
-        const float dz = sp.z - xt[k].z;
-        const bool in = (__builtin_fabsf(dx) < c) && (__builtin_fabsf(dy) < c) &&
-                        (__builtin_fabsf(dz) < c);
-        float z = Zeta<REG>::eval(dx, dy, dz, p.rsigma, p.rc);
-        z = in ? z : 0.f;
-        acc[k][0] = __builtin_fmaf(z, sa.x, acc[k][0]);
-        acc[k][1] = __builtin_fmaf(z, sa.y, acc[k][1]);
-        acc[k][2] = __builtin_fmaf(z, sa.z, acc[k][2]);
+        for (int k = 0; k < T; ++k) {
+          const float dx = sp.x - xt[k].x;
+          const float dy = sp.y - xt[k].y;
+          const float dz = sp.z - xt[k].z;
+          const bool in = (__builtin_fabsf(dx) < c) && (__builtin_fabsf(dy) < c) &&
+                          (__builtin_fabsf(dz) < c);
+          float z = Zeta<REG>::eval(dx, dy, dz, p.rsigma, p.rc);
+          z = in ? z : 0.f;
+          acc[k][0] = __builtin_fmaf(z, sa.x, acc[k][0]);
+          acc[k][1] = __builtin_fmaf(z, sa.y, acc[k][1]);
+          acc[k][2] = __builtin_fmaf(z, sa.z, acc[k][2]);
+        }
       }
     }
     if constexpr (ACC == ACC_COMP) {
 #pragma unroll
       for (int k = 0; k < T; ++k)
 #pragma unroll
-        for (int a = 0; a < 3; ++a) two_sum_add(hi[k][a], lo[k][a], acc[k][a]);
+        for (int a = 0; a < 3; ++a) two_sum_add(sh[k][a], sl[k][a], acc[k][a]);
     }
   }
 
@@ -207,7 +225,7 @@ __global__ __launch_bounds__(WG) void vort_kernel(VortKParams p) {
     double d[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-      d[a] = (ACC == ACC_COMP) ? (double)hi[k][a] + (double)lo[k][a] : (double)acc[k][a];
+      d[a] = (ACC == ACC_COMP) ? (double)sh[k][a] + (double)sl[k][a] : (double)acc[k][a];
     if (p.n_split == 1) {
       const int o = p.tgt_perm ? p.tgt_perm[i] : i;
 #pragma unroll
@@ -244,10 +262,14 @@ __global__ __launch_bounds__(256) void gather_sorted(const float4* __restrict__ 
   a_out[i] = a[j];
 }
 
-// Bounding box of each 256-point tile (one workgroup per tile).
+// Bounding box of each 256-point tile and of each of its 64-point chunks
+// (one workgroup per tile, one wave per chunk).  Lanes past n repeat the
+// last point, which lies in the same tile.
 __global__ __launch_bounds__(256) void tile_boxes(const float4* __restrict__ pv, int n,
                                                   float4* __restrict__ lo,
-                                                  float4* __restrict__ hi) {
+                                                  float4* __restrict__ hi,
+                                                  float4* __restrict__ sub_lo,
+                                                  float4* __restrict__ sub_hi) {
   __shared__ float s[4][6];
   const int i = blockIdx.x * 256 + threadIdx.x;
   const float4 q = pv[i < n ? i : n - 1];
@@ -256,6 +278,9 @@ __global__ __launch_bounds__(256) void tile_boxes(const float4* __restrict__ pv,
   if ((threadIdx.x & 63) == 0) {
     float* w = s[threadIdx.x >> 6];
     w[0] = a0; w[1] = a1; w[2] = a2; w[3] = b0; w[4] = b1; w[5] = b2;
+    const int chunk = blockIdx.x * 4 + (threadIdx.x >> 6);
+    sub_lo[chunk] = make_float4(a0, a1, a2, 0.f);
+    sub_hi[chunk] = make_float4(b0, b1, b2, 0.f);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -321,6 +346,8 @@ VFn vort_pick(int reg, int acc, int T) {
 
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
+constexpr int VORT_SORTED_T = 1;
+
 }  // namespace
 
 double vort_scale(int reg, float sigma) {
@@ -335,12 +362,24 @@ double vort_scale(int reg, float sigma) {
   return k * (1.0 / (4.0 * M_PI * s * s * s));
 }
 
+LaunchCfg vort_default_cfg(int n_src, int n_tgt, int n_cu, bool sorted) {
+  // With tile skipping a smaller target block has a smaller bounding box,
+  // so it skips more source tiles.
+  LaunchCfg c = default_cfg(n_src, n_tgt, n_cu);
+  if (sorted && c.T > VORT_SORTED_T) {
+    c = default_cfg(n_src, n_tgt, n_cu);
+    c.T = VORT_SORTED_T;
+  }
+  return c;
+}
+
 size_t vort_workspace_bytes(const VortJob& j, const LaunchCfg& c, bool sorted) {
   size_t b = 0;
   if (c.n_split > 1) b += align256((size_t)c.n_split * 3 * (size_t)j.n_tgt * sizeof(double));
   if (sorted) {
     const size_t tiles = ((size_t)j.n_src + 255) / 256;
-    b += 2 * align256(sizeof(float4) * (size_t)j.n_src) + 2 * align256(sizeof(float4) * tiles);
+    b += 2 * align256(sizeof(float4) * (size_t)j.n_src) + 2 * align256(sizeof(float4) * tiles) +
+         2 * align256(sizeof(float4) * tiles * 4);
   }
   return b;
 }
@@ -389,13 +428,19 @@ hipError_t launch_vort(const VortJob& j, const LaunchCfg& c, void* ws, size_t ws
     float4* blo = (float4*)w;
     w += align256(sizeof(float4) * (size_t)tiles);
     float4* bhi = (float4*)w;
+    w += align256(sizeof(float4) * (size_t)tiles);
+    float4* clo = (float4*)w;
+    w += align256(sizeof(float4) * (size_t)tiles * 4);
+    float4* chi = (float4*)w;
     hipLaunchKernelGGL(gather_sorted, dim3((j.n_src + 255) / 256), dim3(256), 0, s, j.src_pv,
                        j.src_a, j.src_perm, j.n_src, spv, sa4);
-    hipLaunchKernelGGL(tile_boxes, dim3(tiles), dim3(256), 0, s, spv, j.n_src, blo, bhi);
+    hipLaunchKernelGGL(tile_boxes, dim3(tiles), dim3(256), 0, s, spv, j.n_src, blo, bhi, clo, chi);
     p.src_pv = spv;
     p.src_a = sa4;
     p.box_lo = blo;
     p.box_hi = bhi;
+    p.sub_lo = clo;
+    p.sub_hi = chi;
   }
   const int gx = (j.n_tgt + 256 * c.T - 1) / (256 * c.T);
   hipLaunchKernelGGL(k, dim3(gx, n_split), dim3(256), 0, s, p);

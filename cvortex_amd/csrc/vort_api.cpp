@@ -25,7 +25,7 @@ using Lock = std::lock_guard<std::recursive_mutex>;
 // outside the cut-off box; `same` marks sources == targets (one sort).
 bool vort_core(Runtime& rt, Device& d, const float4* spv, const float4* sa, int ns,
                const float4* tgt, int nt, float* out, int reg, int acc, float sigma, bool same,
-               hipStream_t s) {
+               hipStream_t s, const LaunchCfg* forced = nullptr) {
   VortJob j;
   j.src_pv = spv;
   j.src_a = sa;
@@ -43,7 +43,7 @@ bool vort_core(Runtime& rt, Device& d, const float4* spv, const float4* sa, int 
     j.tgt_perm = same ? j.src_perm : order_points(rt, d, tgt, nt, SLOT_PERM, s);
     if (!j.tgt_perm) return false;
   }
-  const LaunchCfg c = default_cfg(ns, nt, d.n_cu);
+  const LaunchCfg c = forced ? *forced : vort_default_cfg(ns, nt, d.n_cu, sorted);
   const size_t wb = vort_workspace_bytes(j, c, sorted);
   void* ws = wb ? device_buffer(rt, d, SLOT_WORK, wb) : nullptr;
   if (wb && !ws) return false;
@@ -172,7 +172,7 @@ void cvtx_P3D_pedrizzetti_relaxation(cvtx_P3D** particles, const int n, float fd
 
 // ========================= device-resident API =========================
 
-int cvtx_hip_vort_eval(const cvtx_hip_vort_problem* p, void* stream) {
+static int vort_dev(const cvtx_hip_vort_problem* p, const LaunchCfg* forced, void* stream) {
   Runtime& rt = runtime();
   Lock lk(rt.mu);
   if (!p || p->n_src < 0 || p->n_tgt < 0 || (p->n_src > 0 && (!p->src_pv || !p->src_a)) ||
@@ -187,9 +187,22 @@ int cvtx_hip_vort_eval(const cvtx_hip_vort_problem* p, void* stream) {
   const bool same = p->src_pv == p->tgt && p->n_src == p->n_tgt;
   return vort_core(rt, *d, (const float4*)p->src_pv, (const float4*)p->src_a, p->n_src,
                    (const float4*)p->tgt, p->n_tgt, p->vort, p->regularisation, p->accumulate,
-                   p->sigma, same, pick_stream(d, stream))
+                   p->sigma, same, pick_stream(d, stream), forced)
              ? 0
              : -1;
+}
+
+int cvtx_hip_vort_eval(const cvtx_hip_vort_problem* p, void* stream) {
+  return vort_dev(p, nullptr, stream);
+}
+
+int cvtx_hip_vort_eval_shape(const cvtx_hip_vort_problem* p, int T, int n_split, void* stream) {
+  LaunchCfg c;
+  if (!p || !make_launch_shape(T, n_split, p->n_src, &c)) {
+    set_error(runtime(), "invalid launch shape");
+    return -1;
+  }
+  return vort_dev(p, &c, stream);
 }
 
 int cvtx_hip_relax_eval(const float* pv, float* a, int n, float fdt, int regularisation,

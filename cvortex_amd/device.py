@@ -88,8 +88,13 @@ def p3d_eval(prob, stream=None, shape=None):
         raise CvtxError(f"p3d_eval: {last_error()}")
 
 
-def vort_eval(prob, stream=None):
-    if lib().cvtx_hip_vort_eval(ctypes.byref(prob), stream) != 0:
+def vort_eval(prob, stream=None, shape=None):
+    L = lib()
+    if shape is None:
+        st = L.cvtx_hip_vort_eval(ctypes.byref(prob), stream)
+    else:
+        st = L.cvtx_hip_vort_eval_shape(ctypes.byref(prob), shape[0], shape[1], stream)
+    if st != 0:
         raise CvtxError(f"vort_eval: {last_error()}")
 
 

@@ -199,6 +199,8 @@ int cvtx_hip_p2d_eval(const cvtx_hip_p2d_problem* p, void* stream);
 int cvtx_hip_f3d_eval(const cvtx_hip_f3d_problem* p, void* stream);
 
 int cvtx_hip_vort_eval(const cvtx_hip_vort_problem* p, void* stream);
+int cvtx_hip_vort_eval_shape(const cvtx_hip_vort_problem* p, int targets_per_lane, int n_split,
+                             void* stream);
 /* Pedrizzetti relaxation of n device-resident particles (pv, a as above):
  * a is updated in place (its 4th component is kept). */
 int cvtx_hip_relax_eval(const float* pv, float* a, int n, float fdt, int regularisation,
