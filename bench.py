@@ -529,8 +529,63 @@ def bench_relax(args, cv, D, W, o, stream):
     return line
 
 
+# --------------------------------------------------------------------------
+# F3D influence matrix of the 128 x 128 lattice: 16384 bound segments x 16384
+# collocation points (3/4 chord, normal +z)
+# --------------------------------------------------------------------------
+def bench_influence(args, cv, D, W, o, stream):
+    f, _ = W.lattice(n_targets=1)
+    nf = len(f)
+    nx, ny = 128, 128
+    dx, dy = 4.0 / nx, 1.0 / ny
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    pts = np.zeros((nx * ny, 4), np.float32)
+    pts[:, 0] = ii.ravel() * dx + 0.75 * dx
+    pts[:, 1] = (jj.ravel() + 0.5) * dy
+    dirs = np.zeros((nx * ny, 4), np.float32)
+    dirs[:, 2] = 1.0
+    nm = len(pts)
+    fa, fb = D.f3d_soa(f)
+    dfa, dfb, dm, de = (D.DeviceArray.from_host(x) for x in (fa, fb, pts, dirs))
+    out = D.DeviceArray((nm, nf))
+    tm = Timer(stream)
+    log(f"influence matrix {nm} x {nf} ({4.0 * nm * nf / 2**30:.2f} GiB)")
+    ms, k = tm.run(lambda: D.f3d_inf_eval(dfa, dfb, nf, dm, de, nm, out, stream), args.steps,
+                   args.warmup)
+    log(f"  {ms:.3f} ms/step (kernel {k:.3f} ms)")
+    rng = np.random.default_rng(0x5EED + 7)
+    rows = np.sort(rng.choice(nm, size=min(n_err(args, 64) or 64, nm), replace=False)).astype(np.int32)
+    M = out.download()
+    m64 = o.f64_F3D_inf_mtrx(f, pts[:, :3], dirs[:, :3], idx=rows)
+    err = {"rows": int(len(rows)), "matrix": err_stats(M[rows], m64)}
+    cpu = None
+    if not args.no_cpu:
+        sub = rows[:16]
+        best, rv = cpu_time(lambda: o.F3D_inf_mtrx(f, pts[sub, :3], dirs[sub, :3]))
+        import oracle
+        cpu = {"value": len(sub) * float(nf) / best, "unit": "matrix elements/s",
+               "cores": int(o.L.orc_num_threads()), "kind": "port",
+               "sample": f"cvtx_F3D_inf_mtrx on {len(sub)} sampled rows x {nf} filaments, best of 2",
+               "rel_l2_gpu_vs_cpu": oracle.rel_l2(M[sub], rv),
+               "rel_l2_cpu_vs_fp64": oracle.rel_l2(rv, m64[:len(sub)])}
+    elems = float(nm) * nf
+    out_bytes = 4.0 * elems
+    in_bytes = 32.0 * nf + 32.0 * nm
+    achieved = (out_bytes + in_bytes) / (k * 1e-3) / 1e9
+    line = base_line(args, elems / (ms * 1e-3), "matrix elements/s", ms,
+                     {"workload": "F3D influence matrix of the 128x128 bound-segment lattice "
+                                  "([0,4]x[0,1], z=0) at its 3/4-chord collocation points, "
+                                  "normals +z", "n_filaments": nf, "n_points": nm})
+    line.update({"roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0,
+                              "unit": "GB/s", "frac": achieved / 8000.0, "traffic": None,
+                              "kernel_ms": k, "bytes_per_launch": out_bytes + in_bytes},
+                 "rel_err_vs_fp64": err, "cpu_baseline": cpu})
+    return line
+
+
 WORKLOADS = {"ring": bench_ring, "cube": bench_cube, "lattice": bench_lattice,
-             "clouds": bench_clouds, "sheet": bench_sheet, "relax": bench_relax}
+             "clouds": bench_clouds, "sheet": bench_sheet, "relax": bench_relax,
+             "influence": bench_influence}
 
 
 def main():

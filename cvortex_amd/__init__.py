@@ -68,6 +68,8 @@ _SIGNATURES = {
     "cvtx_hip_P3D_M2M_dvort": (_I, [_VP, _I, _VP, _I, _VP, _VP, _F]),
     "cvtx_hip_P3D_M2M_visc_dvort": (_I, [_VP, _I, _VP, _I, _VP, _VP, _F, _F]),
     "cvtx_hip_P3D_M2M_vort": (_I, [_VP, _I, _VP, _I, _VP, _VP, _F]),
+    "cvtx_hip_F3D_inf_mtrx": (_I, [_VP, _I, _VP, _VP, _I, _VP]),
+    "cvtx_F3D_inf_mtrx": (None, [_VP, _I, _VP, _VP, _I, _VP]),
     "cvtx_hip_P3D_pedrizzetti_relaxation": (_I, [_VP, _I, _F, _VP, _F]),
     "cvtx_hip_F3D_M2M_vel": (_I, [_VP, _I, _VP, _I, _VP]),
     "cvtx_hip_F3D_M2M_dvort": (_I, [_VP, _I, _VP, _I, _VP]),
@@ -86,6 +88,7 @@ _SIGNATURES = {
     # device-resident
     "cvtx_hip_p3d_eval": (_I, [_VP, _VP]),
     "cvtx_hip_vort_eval": (_I, [_VP, _VP]),
+    "cvtx_hip_f3d_inf_eval": (_I, [_VP, _VP, _I, _VP, _VP, _I, _VP, _VP]),
     "cvtx_hip_vort_eval_shape": (_I, [_VP, _I, _I, _VP]),
     "cvtx_hip_relax_eval": (_I, [_VP, _VP, _I, _F, _I, _F, _I, _VP]),
     "cvtx_hip_p2d_eval": (_I, [_VP, _VP]),
@@ -262,6 +265,20 @@ def P3D_pedrizzetti_relaxation(particles, fdt, kernel, regularisation_radius):
                                                      float(regularisation_radius)),
            "P3D_pedrizzetti_relaxation")
     return recs
+
+
+def F3D_inf_mtrx(filaments, mes, dirs):
+    """Influence matrix (num_mes, num_filaments): velocity of each filament at
+    each point projected on that point's direction.  libcvtx.h:299-305."""
+    f, keep = _ptrs(filaments, F3D_DTYPE)
+    mes = np.ascontiguousarray(mes, dtype=np.float32).reshape(-1, 3)
+    dirs = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+    if dirs.shape != mes.shape:
+        raise ValueError("one direction per measurement point")
+    out = np.empty((mes.shape[0], len(f)), np.float32)
+    _check(lib().cvtx_hip_F3D_inf_mtrx(_vp(f), len(f), _vp(mes), _vp(dirs), mes.shape[0],
+                                       _vp(out)), "F3D_inf_mtrx")
+    return out
 
 
 def P3D_M2M_dvort(particles, induced, kernel, regularisation_radius):

@@ -245,7 +245,65 @@ F3Fn f3_reduce(int mask) {
     default: return nullptr;
   }
 }
+// Influence matrix (cvtx_F3D_inf_mtrx, reference src/F3D.cpp:204-227):
+//   out[i * n_fil + j] = vel(filament j, point i) . dir_i
+// with the filament's own strength, the velocity term of f3d_kernel (same
+// well-conditioned formula and the same flush of non-finite terms).  Lane
+// j of a workgroup owns filament j of its 256-filament block, so each row
+// segment is written coalesced; the workgroup walks `rows` consecutive
+// points, whose coordinates and directions are the same in every lane.
+__global__ __launch_bounds__(256) void f3d_inf_kernel(const float4* __restrict__ fa,
+                                                      const float4* __restrict__ fb, int n_fil,
+                                                      const float4* __restrict__ mes,
+                                                      const float4* __restrict__ dir, int n_mes,
+                                                      int rows, float* __restrict__ out) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int jj = j < n_fil ? j : n_fil - 1;
+  const float4 a = fa[jj];
+  const float4 b = fb[jj];
+  const float g = a.w * (float)(1.0 / (4.0 * M_PI));
+  const int i0 = blockIdx.y * rows;
+  const int i1 = min(i0 + rows, n_mes);
+  for (int i = i0; i < i1; ++i) {
+    const float4 x = mes[i];
+    const float4 e = dir[i];
+    const float r1x = x.x - a.x, r1y = x.y - a.y, r1z = x.z - a.z;
+    const float r2x = x.x - b.x, r2y = x.y - b.y, r2z = x.z - b.z;
+    const float cx = __builtin_fmaf(r1y, r2z, -r1z * r2y);
+    const float cy = __builtin_fmaf(r1z, r2x, -r1x * r2z);
+    const float cz = __builtin_fmaf(r1x, r2y, -r1y * r2x);
+    const float cc = __builtin_fmaf(cx, cx, __builtin_fmaf(cy, cy, cz * cz));
+    const float n1s = __builtin_fmaf(r1x, r1x, __builtin_fmaf(r1y, r1y, r1z * r1z));
+    const float n2s = __builtin_fmaf(r2x, r2x, __builtin_fmaf(r2y, r2y, r2z * r2z));
+    const float n1 = n1s * __builtin_amdgcn_rsqf(n1s);
+    const float n2 = n2s * __builtin_amdgcn_rsqf(n2s);
+    const float P = n1 * n2;
+    const float sum = n1 + n2;
+    const float d12 = __builtin_fmaf(r1x, r2x, __builtin_fmaf(r1y, r2y, r1z * r2z));
+    const bool same_side = d12 >= 0.f;
+    const float den = same_side ? P * (P + d12) : P * cc;
+    const float num = same_side ? sum : sum * (P - d12);
+    float kk = g * num * __builtin_amdgcn_rcpf(den);
+    kk = (__builtin_fabsf(kk) < __builtin_inff()) ? kk : 0.f;
+    const float v = kk * __builtin_fmaf(cx, e.x, __builtin_fmaf(cy, e.y, cz * e.z));
+    if (j < n_fil) out[(size_t)i * (size_t)n_fil + j] = v;
+  }
+}
+
 }  // namespace
+
+hipError_t launch_f3d_inf(const float4* fa, const float4* fb, int n_fil, const float4* mes,
+                          const float4* dir, int n_mes, float* out, hipStream_t s) {
+  if (n_fil <= 0 || n_mes <= 0) return hipSuccess;
+  if (!fa || !fb || !mes || !dir || !out) return hipErrorInvalidValue;
+  // rows per workgroup: enough workgroups to fill the chip several times
+  const int gx = (n_fil + 255) / 256;
+  int rows = 1;
+  while (rows < 64 && (long long)gx * ((n_mes + 2 * rows - 1) / (2 * rows)) >= 4096) rows *= 2;
+  hipLaunchKernelGGL(f3d_inf_kernel, dim3(gx, (n_mes + rows - 1) / rows), dim3(256), 0, s, fa, fb,
+                     n_fil, mes, dir, n_mes, rows, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_f3d(const F3DJob& j, const LaunchCfg& c, hipStream_t s) {
   if (j.n_tgt <= 0) return hipSuccess;

@@ -111,6 +111,10 @@ LaunchCfg default_cfg(int n_src, int n_tgt, int n_cu);
 hipError_t launch_p3d(const P3DJob& j, const LaunchCfg& c, hipStream_t s);
 hipError_t launch_p2d(const P2DJob& j, const LaunchCfg& c, hipStream_t s);
 hipError_t launch_f3d(const F3DJob& j, const LaunchCfg& c, hipStream_t s);
+// F3D influence matrix: out[i * n_fil + j] = vel(filament j, mes_i) . dir_i
+// (fa, fb as in F3DJob; mes, dir float4 {x, y, z, -}).
+hipError_t launch_f3d_inf(const float4* fa, const float4* fb, int n_fil, const float4* mes,
+                          const float4* dir, int n_mes, float* out, hipStream_t s);
 // Vorticity field: workspace of vort_workspace_bytes (source partitions;
 // sorted source copies and tile boxes when j.src_perm is set).
 size_t vort_workspace_bytes(const VortJob& j, const LaunchCfg& c, bool sorted);

@@ -98,6 +98,13 @@ def vort_eval(prob, stream=None, shape=None):
         raise CvtxError(f"vort_eval: {last_error()}")
 
 
+def f3d_inf_eval(fa, fb, n_fil, mes, dirs, n_mes, out, stream=None):
+    """Influence matrix of device filaments into out (n_mes, n_fil)."""
+    if lib().cvtx_hip_f3d_inf_eval(fa.addr, fb.addr, n_fil, mes.addr, dirs.addr, n_mes, out.addr,
+                                   stream) != 0:
+        raise CvtxError(f"f3d_inf_eval: {last_error()}")
+
+
 def relax_eval(pv, a, n, fdt, kernel, sigma, accumulate=1, stream=None):
     """In-place Pedrizzetti relaxation of device strengths `a` (float4 rows)."""
     reg = REG_ID[kernel] if isinstance(kernel, str) else int(kernel)

@@ -172,6 +172,19 @@ CVTX_EXPORT void cvtx_F3D_M2M_dvort(
 	const int num_induced,
 	bsv_V3f *result_array);
 
+/* Straight vortex filaments: influence matrix, row-major num_mes x
+ * num_filaments: result_matrix[i * num_filaments + j] is the velocity of
+ * filament j (with its own strength, as the reference code computes it)
+ * at mes_start[i], projected on dir_start[i].  Reference libcvtx.h:299-305,
+ * src/F3D.cpp:204-227. */
+CVTX_EXPORT void cvtx_F3D_inf_mtrx(
+	const cvtx_F3D **array_start,
+	const int num_filaments,
+	const bsv_V3f *mes_start,
+	const bsv_V3f *dir_start,
+	const int num_mes,
+	float *result_matrix);
+
 /* 2D vortex particles: velocity induced at points.  Reference
  * libcvtx.h:329-336, src/P2D.cpp:141-162. */
 CVTX_EXPORT void cvtx_P2D_M2M_vel(

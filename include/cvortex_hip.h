@@ -96,6 +96,12 @@ int cvtx_hip_P3D_pedrizzetti_relaxation(cvtx_P3D** input_array_start, int n_inpu
                                         float fdt, const cvtx_VortFunc* kernel,
                                         float regularisation_radius);
 
+/* cvtx_F3D_inf_mtrx on the device (the reference computes it on the CPU
+ * only, src/F3D.cpp:204-227) */
+int cvtx_hip_F3D_inf_mtrx(const cvtx_F3D** array_start, int num_filaments,
+                          const bsv_V3f* mes_start, const bsv_V3f* dir_start, int num_mes,
+                          float* result_matrix);
+
 /* ---- 2. fused host-array entry points (no reference counterpart) ----
  * Any of the result pointers may be NULL (that output is not computed). */
 int cvtx_hip_P3D_M2M_fused(const cvtx_P3D** array_start, int num_particles,
@@ -201,6 +207,11 @@ int cvtx_hip_f3d_eval(const cvtx_hip_f3d_problem* p, void* stream);
 int cvtx_hip_vort_eval(const cvtx_hip_vort_problem* p, void* stream);
 int cvtx_hip_vort_eval_shape(const cvtx_hip_vort_problem* p, int targets_per_lane, int n_split,
                              void* stream);
+/* Influence matrix of device-resident filaments (fa = float4 {start,
+ * strength}, fb = float4 {end, -}) at points mes with directions dir (float4
+ * {x, y, z, -} each): out[i * n_fil + j], float[n_mes * n_fil]. */
+int cvtx_hip_f3d_inf_eval(const float* fa, const float* fb, int n_fil, const float* mes,
+                          const float* dir, int n_mes, float* out, void* stream);
 /* Pedrizzetti relaxation of n device-resident particles (pv, a as above):
  * a is updated in place (its 4th component is kept). */
 int cvtx_hip_relax_eval(const float* pv, float* a, int n, float fdt, int regularisation,

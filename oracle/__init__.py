@@ -49,6 +49,8 @@ _SIGS = {
     "orc_F3D_S2S_dvort": (V3f, [ctypes.POINTER(F3D), ctypes.POINTER(P3D)]),
     "orc_F3D_M2M_vel": (None, [_VP, _I, _VP, _I, _VP]),
     "orc_F3D_M2M_dvort": (None, [_VP, _I, _VP, _I, _VP]),
+    "orc_F3D_inf_mtrx": (None, [_VP, _I, _VP, _VP, _I, _VP]),
+    "orc64_F3D_inf_mtrx": (None, [_VP, _I, _VP, _VP, _VP, _I, _VP]),
     "orc_P2D_S2S_vel": (V2f, [ctypes.POINTER(P2D), V2f, _I, _F]),
     "orc_P2D_S2S_visc_dvort": (_F, [ctypes.POINTER(P2D), ctypes.POINTER(P2D), _I, _F, _F]),
     "orc_P2D_M2M_vel": (None, [_VP, _I, _VP, _I, _VP, _I, _F]),
@@ -214,6 +216,24 @@ class Oracle:
         q, k2 = pointer_array(as_records(induced, P3D_DTYPE))
         out = np.empty((len(q), 3), np.float32)
         self.L.orc_F3D_M2M_dvort(_vp(f), len(f), _vp(q), len(q), _vp(out))
+        return out
+
+    def F3D_inf_mtrx(self, fils, mes, dirs):
+        f, keep = pointer_array(as_records(fils, F3D_DTYPE))
+        mes = np.ascontiguousarray(mes, np.float32).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.empty((len(mes), len(f)), np.float32)
+        self.L.orc_F3D_inf_mtrx(_vp(f), len(f), _vp(mes), _vp(dirs), len(mes), _vp(out))
+        return out
+
+    def f64_F3D_inf_mtrx(self, fils, mes, dirs, idx=None):
+        f, keep = pointer_array(as_records(fils, F3D_DTYPE))
+        mes = np.ascontiguousarray(mes, np.float32).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        idx = np.arange(len(mes), dtype=np.int32) if idx is None else np.ascontiguousarray(idx, np.int32)
+        out = np.zeros((len(idx), len(f)))
+        self.L.orc64_F3D_inf_mtrx(_vp(f), len(f), _vp(mes), _vp(dirs), _vp(idx), len(idx),
+                                  _vp(out))
         return out
 
     def P2D_M2M_vel(self, particles, mes, kernel, sigma):

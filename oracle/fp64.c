@@ -293,6 +293,22 @@ void orc64_F3D_M2M_vel_pts(const cvtx_F3D** src, int nf, const bsv_V3f* mes,
 		f3d_one(src, nf, mes[tgt_idx[k]].x, NULL, vel + 3 * k, NULL);
 }
 
+/* influence matrix, reference src/F3D.cpp:204-227: each element is one
+ * filament's velocity (f3d_one over a single filament) dotted with dir */
+void orc64_F3D_inf_mtrx(const cvtx_F3D** src, int nf, const bsv_V3f* mes, const bsv_V3f* dir,
+                        const int* tgt_idx, int n_idx, double* out) {
+	long k;
+#pragma omp parallel for schedule(dynamic, 4)
+	for (k = 0; k < n_idx; ++k) {
+		const int i = tgt_idx[k];
+		for (int j = 0; j < nf; ++j) {
+			double v[3];
+			f3d_one(src + j, 1, mes[i].x, NULL, v, NULL);
+			out[k * (long)nf + j] = v[0] * dir[i].x[0] + v[1] * dir[i].x[1] + v[2] * dir[i].x[2];
+		}
+	}
+}
+
 /* ------------------------------- P2D ------------------------------- */
 
 static void p2d_one(const cvtx_P2D** src, int ns, const float* xt_f, float at, float At,

@@ -62,6 +62,8 @@ bsv_V3f orc_F3D_S2S_vel(const cvtx_F3D* self, bsv_V3f mes);
 bsv_V3f orc_F3D_S2S_dvort(const cvtx_F3D* self, const cvtx_P3D* induced);
 void orc_F3D_M2M_vel(const cvtx_F3D** src, int nf, const bsv_V3f* mes, int nm, bsv_V3f* out);
 void orc_F3D_M2M_dvort(const cvtx_F3D** src, int nf, const cvtx_P3D** ind, int ni, bsv_V3f* out);
+void orc_F3D_inf_mtrx(const cvtx_F3D** src, int nf, const bsv_V3f* mes, const bsv_V3f* dir,
+                      int nm, float* out);
 
 bsv_V2f orc_P2D_S2S_vel(const cvtx_P2D* self, bsv_V2f mes, int reg, float sigma);
 float orc_P2D_S2S_visc_dvort(const cvtx_P2D* self, const cvtx_P2D* induced, int reg,
@@ -93,6 +95,9 @@ void orc64_P3D_pedrizzetti(const cvtx_P3D** p, int n, double fdt, int reg, doubl
                            int gauss_mode, double* alpha_out);
 void orc64_F3D_M2M(const cvtx_F3D** src, int nf, const cvtx_P3D** ind,
                    const int* tgt_idx, int n_idx, double* vel, double* dvort);
+/* influence matrix in double: out[k * nf + j] for point tgt_idx[k] */
+void orc64_F3D_inf_mtrx(const cvtx_F3D** src, int nf, const bsv_V3f* mes, const bsv_V3f* dir,
+                        const int* tgt_idx, int n_idx, double* out);
 void orc64_F3D_M2M_vel_pts(const cvtx_F3D** src, int nf, const bsv_V3f* mes,
                            const int* tgt_idx, int n_idx, double* vel);
 void orc64_P2D_M2M(const cvtx_P2D** src, int ns, const cvtx_P2D** ind,

@@ -466,6 +466,21 @@ void orc_F3D_M2M_dvort(const cvtx_F3D** src, int nf, const cvtx_P3D** ind, int n
 	}
 }
 
+/* reference src/F3D.cpp:204-227: row-major, row i = measurement point i,
+ * result[i * nf + j] = S2S_vel(filament j, mes_i) . dir_i */
+void orc_F3D_inf_mtrx(const cvtx_F3D** src, int nf, const bsv_V3f* mes, const bsv_V3f* dir,
+                      int nm, float* out) {
+	long i;
+#pragma omp parallel for schedule(static)
+	for (i = 0; i < nm; ++i) {
+		for (long j = 0; j < nf; ++j) {
+			bsv_V3f v = orc_F3D_S2S_vel(src[j], mes[i]);
+			long at = ORC_MUT(16) ? j * (long)nm + i : i * (long)nf + j;
+			out[at] = v3_dot(v, dir[i]);
+		}
+	}
+}
+
 /* ------------------------------------------------------------------ */
 /* P2D: reference src/P2D.cpp:49-276                                   */
 /* ------------------------------------------------------------------ */

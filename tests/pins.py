@@ -539,7 +539,33 @@ def relaxation_invariants(o):
     return f
 
 
+# ---------------------------------------------------------------------------
+# J. F3D influence matrix, reference src/F3D.cpp:204-227: row i (point i)
+#    summed over filaments is the total velocity at point i (already pinned
+#    by quadrature) projected on dir_i; the layout is row-major by point.
+# ---------------------------------------------------------------------------
+def influence_matrix_rows(o):
+    f = []
+    fl = W.random_f3d(37, 27, box=1.0, strength=1.0)
+    pts = W.random_p3d(23, 28, box=1.0, vort=1.0)["coord"]
+    g = np.random.default_rng(3).normal(size=(23, 3)).astype(np.float32)
+    M = o.F3D_inf_mtrx(fl, pts, g).astype(np.float64)
+    if M.shape != (23, 37):
+        return ["influence matrix shape"]
+    tot = o.F3D_M2M_vel(fl, pts).astype(np.float64)
+    want = (tot * g.astype(np.float64)).sum(axis=1)
+    if np.abs(M.sum(axis=1) - want).max() > 1e-5 * np.abs(want).max():
+        f.append("influence matrix rows do not sum to the projected velocity")
+    for j in (0, 17, 36):
+        vj = o.F3D_M2M_vel(fl[j:j + 1], pts).astype(np.float64)
+        if np.abs(M[:, j] - (vj * g).sum(axis=1)).max() > 1e-5 * np.abs(M[:, j]).max():
+            f.append(f"influence matrix column {j} is not filament {j}'s velocity")
+    if orc.rel_l2(M, o.f64_F3D_inf_mtrx(fl, pts, g)) > 5e-6:
+        f.append("influence matrix FP32 vs FP64")
+    return f
+
+
 ALL_CHECKS = (kat_vortfunc, moments_3d, g_cumulative_zeta, kernels_2d, particle_invariants,
               biot_savart_pair, dvort_transpose_fd, pse_conservation, f3d_vel_quadrature,
               f3d_dvort_structure, p2d_point_vortex, ring_centre, refcpu_vs_fp64,
-              vort_box_cutoff, relaxation_invariants)
+              vort_box_cutoff, relaxation_invariants, influence_matrix_rows)

@@ -33,6 +33,7 @@ EXPECTED = {
     13: "vort_box_cutoff",      # vorticity cut-off 5 sigma -> 4 sigma
     14: "vort_box_cutoff",      # vorticity cut-off box -> sphere
     15: "relaxation_invariants",  # relaxation keeps fdt alpha instead of (1 - fdt) alpha
+    16: "influence_matrix_rows",  # influence matrix stored column-major
 }
 
 
