@@ -18,7 +18,7 @@ import os
 
 import numpy as np
 
-from .types import (F3D_DTYPE, P2D_DTYPE, P3D_DTYPE, REG_ID, REG_NAMES, VortFunc,
+from .types import (F3D_DTYPE, P2D_DTYPE, P3D_DTYPE, REG_ID, REG_NAMES, V2f, V3f, VortFunc,
                     as_records, pointer_array)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcvortex.so")
@@ -55,6 +55,29 @@ _SIGNATURES = {
     "cvtx_VortFunc_gaussian": (VortFunc, []),
     # public M2M (void)
     "cvtx_P3D_M2M_vel": (None, [_VP, _I, _VP, _I, _VP, _VP, _F]),
+    # single-source / single-target forms (records by pointer, points by value)
+    "cvtx_P3D_S2S_vel": (V3f, [_VP, V3f, _VP, _F]),
+    "cvtx_P3D_S2S_dvort": (V3f, [_VP, _VP, _VP, _F]),
+    "cvtx_P3D_S2S_visc_dvort": (V3f, [_VP, _VP, _VP, _F, _F]),
+    "cvtx_P3D_S2S_vort": (V3f, [_VP, V3f, _VP, _F]),
+    "cvtx_P3D_S2M_vel": (None, [_VP, _VP, _I, _VP, _VP, _F]),
+    "cvtx_P3D_S2M_dvort": (None, [_VP, _VP, _I, _VP, _VP, _F]),
+    "cvtx_P3D_S2M_visc_dvort": (None, [_VP, _VP, _I, _VP, _VP, _F, _F]),
+    "cvtx_P3D_S2M_vort": (None, [_VP, _VP, _I, _VP, _VP, _F]),
+    "cvtx_P3D_M2S_vel": (V3f, [_VP, _I, V3f, _VP, _F]),
+    "cvtx_P3D_M2S_dvort": (V3f, [_VP, _I, _VP, _VP, _F]),
+    "cvtx_P3D_M2S_visc_dvort": (V3f, [_VP, _I, _VP, _VP, _F, _F]),
+    "cvtx_P3D_M2S_vort": (V3f, [_VP, _I, V3f, _VP, _F]),
+    "cvtx_F3D_S2S_vel": (V3f, [_VP, V3f]),
+    "cvtx_F3D_S2S_dvort": (V3f, [_VP, _VP]),
+    "cvtx_F3D_M2S_vel": (V3f, [_VP, _I, V3f]),
+    "cvtx_F3D_M2S_dvort": (V3f, [_VP, _I, _VP]),
+    "cvtx_P2D_S2S_vel": (V2f, [_VP, V2f, _VP, _F]),
+    "cvtx_P2D_S2M_vel": (None, [_VP, _VP, _I, _VP, _VP, _F]),
+    "cvtx_P2D_M2S_vel": (V2f, [_VP, _I, V2f, _VP, _F]),
+    "cvtx_P2D_S2S_visc_dvort": (_F, [_VP, _VP, _VP, _F, _F]),
+    "cvtx_P2D_S2M_visc_dvort": (None, [_VP, _VP, _I, _VP, _VP, _F, _F]),
+    "cvtx_P2D_M2S_visc_dvort": (_F, [_VP, _I, _VP, _VP, _F, _F]),
     "cvtx_P3D_M2M_vort": (None, [_VP, _I, _VP, _I, _VP, _VP, _F]),
     "cvtx_P3D_pedrizzetti_relaxation": (None, [_VP, _I, _F, _VP, _F]),
     "cvtx_P3D_M2M_dvort": (None, [_VP, _I, _VP, _I, _VP, _VP, _F]),
@@ -389,3 +412,168 @@ def P2D_M2M_fused(particles, induced, kernel, regularisation_radius, kinematic_v
 
 
 __all__ = [n for n in dir() if not n.startswith("_")] + ["REG_ID", "REG_NAMES"]
+
+
+# --------------- single-source (S2S, S2M) / single-target (M2S) ---------------
+# The reference's small entry points (libcvtx.h:125-211, 266-283, 306-360),
+# called through the public C ABI: results are NaN (with a line on stderr)
+# when the call fails.
+
+def _rec(r, dtype):
+    """Pointer to one record (kept alive by the returned array)."""
+    a = as_records(np.atleast_1d(np.asarray(r)), dtype)[:1].copy()
+    return ctypes.c_void_p(a.ctypes.data), a
+
+
+def _v3(x):
+    return V3f((ctypes.c_float * 3)(*np.asarray(x, np.float32).ravel()[:3]))
+
+
+def _v2(x):
+    return V2f((ctypes.c_float * 2)(*np.asarray(x, np.float32).ravel()[:2]))
+
+
+def _arr(v, n):
+    return np.array(v.x[:n], np.float32)
+
+
+def P3D_S2S_vel(p, mes, kernel, sigma):
+    rp, k1 = _rec(p, P3D_DTYPE)
+    k, kp = _kref(kernel)
+    return _arr(lib().cvtx_P3D_S2S_vel(rp, _v3(mes), kp, float(sigma)), 3)
+
+
+def P3D_S2S_dvort(p, ind, kernel, sigma):
+    rp, k1 = _rec(p, P3D_DTYPE)
+    ri, k2 = _rec(ind, P3D_DTYPE)
+    k, kp = _kref(kernel)
+    return _arr(lib().cvtx_P3D_S2S_dvort(rp, ri, kp, float(sigma)), 3)
+
+
+def P3D_S2S_visc_dvort(p, ind, kernel, sigma, nu):
+    rp, k1 = _rec(p, P3D_DTYPE)
+    ri, k2 = _rec(ind, P3D_DTYPE)
+    k, kp = _kref(kernel)
+    return _arr(lib().cvtx_P3D_S2S_visc_dvort(rp, ri, kp, float(sigma), float(nu)), 3)
+
+
+def P3D_S2S_vort(p, mes, kernel, sigma):
+    rp, k1 = _rec(p, P3D_DTYPE)
+    k, kp = _kref(kernel)
+    return _arr(lib().cvtx_P3D_S2S_vort(rp, _v3(mes), kp, float(sigma)), 3)
+
+
+def _s2m_points(fn, p, mes, kernel, sigma, dtype, dim):
+    rp, k1 = _rec(p, dtype)
+    mes = np.ascontiguousarray(mes, np.float32).reshape(-1, dim)
+    out = np.empty_like(mes)
+    k, kp = _kref(kernel)
+    fn(rp, _vp(mes), mes.shape[0], _vp(out), kp, float(sigma))
+    return out
+
+
+def _s2m_records(fn, p, induced, kernel, dtype, width, *extra):
+    rp, k1 = _rec(p, dtype)
+    q, k2 = _ptrs(induced, dtype)
+    out = np.empty((len(q), width) if width > 1 else len(q), np.float32)
+    k, kp = _kref(kernel)
+    fn(rp, _vp(q), len(q), _vp(out), kp, *[float(x) for x in extra])
+    return out
+
+
+def P3D_S2M_vel(p, mes, kernel, sigma):
+    return _s2m_points(lib().cvtx_P3D_S2M_vel, p, mes, kernel, sigma, P3D_DTYPE, 3)
+
+
+def P3D_S2M_vort(p, mes, kernel, sigma):
+    return _s2m_points(lib().cvtx_P3D_S2M_vort, p, mes, kernel, sigma, P3D_DTYPE, 3)
+
+
+def P3D_S2M_dvort(p, induced, kernel, sigma):
+    return _s2m_records(lib().cvtx_P3D_S2M_dvort, p, induced, kernel, P3D_DTYPE, 3, sigma)
+
+
+def P3D_S2M_visc_dvort(p, induced, kernel, sigma, nu):
+    return _s2m_records(lib().cvtx_P3D_S2M_visc_dvort, p, induced, kernel, P3D_DTYPE, 3, sigma, nu)
+
+
+def P3D_M2S_vel(particles, mes, kernel, sigma):
+    p, keep = _ptrs(particles, P3D_DTYPE)
+    k, kp = _kref(kernel)
+    return _arr(lib().cvtx_P3D_M2S_vel(_vp(p), len(p), _v3(mes), kp, float(sigma)), 3)
+
+
+def P3D_M2S_vort(particles, mes, kernel, sigma):
+    p, keep = _ptrs(particles, P3D_DTYPE)
+    k, kp = _kref(kernel)
+    return _arr(lib().cvtx_P3D_M2S_vort(_vp(p), len(p), _v3(mes), kp, float(sigma)), 3)
+
+
+def P3D_M2S_dvort(particles, ind, kernel, sigma):
+    p, keep = _ptrs(particles, P3D_DTYPE)
+    ri, k2 = _rec(ind, P3D_DTYPE)
+    k, kp = _kref(kernel)
+    return _arr(lib().cvtx_P3D_M2S_dvort(_vp(p), len(p), ri, kp, float(sigma)), 3)
+
+
+def P3D_M2S_visc_dvort(particles, ind, kernel, sigma, nu):
+    p, keep = _ptrs(particles, P3D_DTYPE)
+    ri, k2 = _rec(ind, P3D_DTYPE)
+    k, kp = _kref(kernel)
+    return _arr(lib().cvtx_P3D_M2S_visc_dvort(_vp(p), len(p), ri, kp, float(sigma), float(nu)), 3)
+
+
+def F3D_S2S_vel(f, mes):
+    rf, k1 = _rec(f, F3D_DTYPE)
+    return _arr(lib().cvtx_F3D_S2S_vel(rf, _v3(mes)), 3)
+
+
+def F3D_S2S_dvort(f, ind):
+    rf, k1 = _rec(f, F3D_DTYPE)
+    ri, k2 = _rec(ind, P3D_DTYPE)
+    return _arr(lib().cvtx_F3D_S2S_dvort(rf, ri), 3)
+
+
+def F3D_M2S_vel(fils, mes):
+    f, keep = _ptrs(fils, F3D_DTYPE)
+    return _arr(lib().cvtx_F3D_M2S_vel(_vp(f), len(f), _v3(mes)), 3)
+
+
+def F3D_M2S_dvort(fils, ind):
+    f, keep = _ptrs(fils, F3D_DTYPE)
+    ri, k2 = _rec(ind, P3D_DTYPE)
+    return _arr(lib().cvtx_F3D_M2S_dvort(_vp(f), len(f), ri), 3)
+
+
+def P2D_S2S_vel(p, mes, kernel, sigma):
+    rp, k1 = _rec(p, P2D_DTYPE)
+    k, kp = _kref(kernel)
+    return _arr(lib().cvtx_P2D_S2S_vel(rp, _v2(mes), kp, float(sigma)), 2)
+
+
+def P2D_S2M_vel(p, mes, kernel, sigma):
+    return _s2m_points(lib().cvtx_P2D_S2M_vel, p, mes, kernel, sigma, P2D_DTYPE, 2)
+
+
+def P2D_M2S_vel(particles, mes, kernel, sigma):
+    p, keep = _ptrs(particles, P2D_DTYPE)
+    k, kp = _kref(kernel)
+    return _arr(lib().cvtx_P2D_M2S_vel(_vp(p), len(p), _v2(mes), kp, float(sigma)), 2)
+
+
+def P2D_S2S_visc_dvort(p, ind, kernel, sigma, nu):
+    rp, k1 = _rec(p, P2D_DTYPE)
+    ri, k2 = _rec(ind, P2D_DTYPE)
+    k, kp = _kref(kernel)
+    return float(lib().cvtx_P2D_S2S_visc_dvort(rp, ri, kp, float(sigma), float(nu)))
+
+
+def P2D_S2M_visc_dvort(p, induced, kernel, sigma, nu):
+    return _s2m_records(lib().cvtx_P2D_S2M_visc_dvort, p, induced, kernel, P2D_DTYPE, 1, sigma, nu)
+
+
+def P2D_M2S_visc_dvort(particles, ind, kernel, sigma, nu):
+    p, keep = _ptrs(particles, P2D_DTYPE)
+    ri, k2 = _rec(ind, P2D_DTYPE)
+    k, kp = _kref(kernel)
+    return float(lib().cvtx_P2D_M2S_visc_dvort(_vp(p), len(p), ri, kp, float(sigma), float(nu)))

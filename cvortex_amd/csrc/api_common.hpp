@@ -32,6 +32,10 @@ void report_failure(const char* fn, float* out, long n);
 hipStream_t pick_stream(Device* d, void* s);
 // Morton permutation of n points into runtime slot `slot` (nullptr on failure)
 int* order_points(Runtime& rt, Device& d, const float4* pts, int n, int slot, hipStream_t s);
+// Vorticity field of host records at host points (vort_api.cpp); cutoff =
+// false for the reference's single-source forms (no 5 sigma box).
+int vort_host(const cvtx_P3D** src, int ns, const bsv_V3f* mes, int nt, float* out,
+              const cvtx_VortFunc* vf, float sigma, bool cutoff);
 // explicit launch shape: T in {1, 2, 4}, n_split >= 1 source partitions
 // (rounded to whole 256-source tiles); false for an invalid shape
 bool make_launch_shape(int T, int n_split, int n_src, LaunchCfg* c);

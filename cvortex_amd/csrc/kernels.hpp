@@ -91,6 +91,7 @@ struct VortJob {
   int reg = REG_WINCKELMANS;
   int acc = ACC_COMP;
   float sigma = 1.f;
+  bool cutoff = true;              // false: no 5 sigma box (single-source forms)
 };
 
 // number of FP64 partial accumulators per target for a mask

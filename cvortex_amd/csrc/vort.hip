@@ -34,16 +34,20 @@ namespace {
 template <int REG>
 struct Zeta;
 
+// eval(dx, dy, dz, rsigma, sigma, divide, consts): rsigma = 1/sigma and
+// sigma feed only the planetary core test (rho = |d| rsigma, or |d| / sigma
+// when `divide`, as the reference's two vorticity forms compute it).
 template <>
 struct Zeta<REG_SINGULAR> {  // zeta = 0
-  static __device__ __forceinline__ float eval(float, float, float, float, const RegConsts&) {
+  static __device__ __forceinline__ float eval(float, float, float, float, float, bool,
+                                               const RegConsts&) {
     return 0.f;
   }
 };
 
 template <>
 struct Zeta<REG_WINCKELMANS> {  // 7.5 (rho^2 + 1)^-3.5 -> y^7, y = (q + 1)^-1/2
-  static __device__ __forceinline__ float eval(float dx, float dy, float dz, float,
+  static __device__ __forceinline__ float eval(float dx, float dy, float dz, float, float, bool,
                                                const RegConsts& k) {
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
     const float y = __builtin_amdgcn_rsqf(__builtin_fmaf(r2, k.inv_s2, 1.f));
@@ -56,19 +60,21 @@ struct Zeta<REG_WINCKELMANS> {  // 7.5 (rho^2 + 1)^-3.5 -> y^7, y = (q + 1)^-1/2
 template <>
 struct Zeta<REG_PLANETARY> {  // 3 inside the core, 0 outside -> 1 / 0
   // The core test is a discontinuity, so rho is formed with the reference's
-  // FP32 operations (src/P3D.cpp:314-315): |d| = sqrt(dx dx + dy dy + dz dz)
-  // rounded at every step, then rho = |d| * (1/sigma).
+  // FP32 operations: |d| = sqrt(dx dx + dy dy + dz dz) rounded at every
+  // step, then rho = |d| * (1/sigma) (src/P3D.cpp:314-315) or |d| / sigma
+  // (src/P3D.cpp:154-155).
   static __device__ __forceinline__ float eval(float dx, float dy, float dz, float rsigma,
-                                               const RegConsts&) {
+                                               float sigma, bool divide, const RegConsts&) {
     const float r2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
-    const float rho = __fmul_rn(__fsqrt_rn(r2), rsigma);
+    const float r = __fsqrt_rn(r2);
+    const float rho = divide ? __fdiv_rn(r, sigma) : __fmul_rn(r, rsigma);
     return rho < 1.f ? 1.f : 0.f;
   }
 };
 
 template <>
 struct Zeta<REG_GAUSSIAN> {  // sqrt(2/pi) exp(-rho^2 / 2) -> 2^(q c1)
-  static __device__ __forceinline__ float eval(float dx, float dy, float dz, float,
+  static __device__ __forceinline__ float eval(float dx, float dy, float dz, float, float, bool,
                                                const RegConsts& k) {
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
     return __builtin_amdgcn_exp2f(r2 * k.inv_s2 * k.c1);
@@ -87,8 +93,10 @@ struct VortKParams {
   float* __restrict__ out;
   double* __restrict__ partial;
   int n_src, n_tgt, split_len, n_split;
-  float cutoff;  // 5 sigma in FP32, as the reference forms it
+  float cutoff;  // 5 sigma in FP32, as the reference forms it (+inf: no cut-off)
   float rsigma;  // 1 / sigma in FP32
+  float sigma;
+  int divide;    // planetary rho = |d| / sigma instead of |d| * rsigma
   RegConsts rc;
   double scale;  // zeta constant / (4 pi sigma^3)
 };
@@ -202,7 +210,7 @@ __global__ __launch_bounds__(WG) void vort_kernel(VortKParams p) {
           const float dz = sp.z - xt[k].z;
           const bool in = (__builtin_fabsf(dx) < c) && (__builtin_fabsf(dy) < c) &&
                           (__builtin_fabsf(dz) < c);
-          float z = Zeta<REG>::eval(dx, dy, dz, p.rsigma, p.rc);
+          float z = Zeta<REG>::eval(dx, dy, dz, p.rsigma, p.sigma, p.divide != 0, p.rc);
           z = in ? z : 0.f;
           acc[k][0] = __builtin_fmaf(z, sa.x, acc[k][0]);
           acc[k][1] = __builtin_fmaf(z, sa.y, acc[k][1]);
@@ -401,8 +409,12 @@ hipError_t launch_vort(const VortJob& j, const LaunchCfg& c, void* ws, size_t ws
   p.out = j.out;
   p.n_src = j.n_src;
   p.n_tgt = j.n_tgt;
-  p.cutoff = 5.f * j.sigma;
+  // the reference's single-source forms (S2S, S2M) have no cut-off and
+  // divide by sigma (src/P3D.cpp:146-161); M2S and M2M cut at 5 sigma
+  p.cutoff = j.cutoff ? 5.f * j.sigma : __builtin_inff();
   p.rsigma = 1.f / j.sigma;
+  p.sigma = j.sigma;
+  p.divide = j.cutoff ? 0 : 1;
   const double sa = (double)j.sigma;
   p.rc.s2 = (float)(sa * sa);
   p.rc.inv_s2 = (float)(1.0 / (sa * sa));

@@ -25,7 +25,7 @@ using Lock = std::lock_guard<std::recursive_mutex>;
 // outside the cut-off box; `same` marks sources == targets (one sort).
 bool vort_core(Runtime& rt, Device& d, const float4* spv, const float4* sa, int ns,
                const float4* tgt, int nt, float* out, int reg, int acc, float sigma, bool same,
-               hipStream_t s, const LaunchCfg* forced = nullptr) {
+               hipStream_t s, const LaunchCfg* forced = nullptr, bool cutoff = true) {
   VortJob j;
   j.src_pv = spv;
   j.src_a = sa;
@@ -36,7 +36,8 @@ bool vort_core(Runtime& rt, Device& d, const float4* spv, const float4* sa, int 
   j.reg = reg;
   j.acc = acc;
   j.sigma = sigma;
-  const bool sorted = rt.order_targets && vort_wants_sort(ns, nt);
+  j.cutoff = cutoff;
+  const bool sorted = cutoff && rt.order_targets && vort_wants_sort(ns, nt);
   if (sorted) {
     j.src_perm = order_points(rt, d, spv, ns, SLOT_PERM2, s);
     if (!j.src_perm) return false;
@@ -61,8 +62,10 @@ int regularisation_of(Runtime& rt, const cvtx_VortFunc* vf) {
   return reg;
 }
 
-int vort_host(const cvtx_P3D** src, int ns, const bsv_V3f* mes, int nt, float* out,
-              const cvtx_VortFunc* vf, float sigma) {
+}  // namespace
+
+int cvtx::vort_host(const cvtx_P3D** src, int ns, const bsv_V3f* mes, int nt, float* out,
+                    const cvtx_VortFunc* vf, float sigma, bool cutoff) {
   Runtime& rt = runtime();
   Lock lk(rt.mu);
   const int reg = regularisation_of(rt, vf);
@@ -93,13 +96,15 @@ int vort_host(const cvtx_P3D** src, int ns, const bsv_V3f* mes, int nt, float* o
   if (!h2d(rt, *d, d_sa, h_sa, sizeof(float4) * (size_t)ns)) return -1;
   if (!h2d(rt, *d, d_t, h_t, tb)) return -1;
   if (!vort_core(rt, *d, d_spv, d_sa, ns, d_t, nt, d_out, reg, rt.acc_mode, sigma, false,
-                 d->stream))
+                 d->stream, nullptr, cutoff))
     return -1;
   if (!d2h(rt, *d, h_out, d_out, ob)) return -1;
   if (!check_hip(rt, hipStreamSynchronize(d->stream), "vorticity kernels")) return -1;
   std::memcpy(out, h_out, ob);
   return 0;
 }
+
+namespace {
 
 int relax_host(cvtx_P3D** p, int n, float fdt, const cvtx_VortFunc* vf, float sigma) {
   Runtime& rt = runtime();
@@ -143,7 +148,7 @@ extern "C" {
 
 int cvtx_hip_P3D_M2M_vort(const cvtx_P3D** src, int ns, const bsv_V3f* mes, int nm, bsv_V3f* out,
                           const cvtx_VortFunc* k, float sigma) {
-  return vort_host(src, ns, mes, nm, (float*)out, k, sigma);
+  return vort_host(src, ns, mes, nm, (float*)out, k, sigma, true);
 }
 
 int cvtx_hip_P3D_pedrizzetti_relaxation(cvtx_P3D** particles, int n, float fdt,

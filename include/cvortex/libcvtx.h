@@ -208,6 +208,69 @@ CVTX_EXPORT void cvtx_P2D_M2M_visc_dvort(
 	float regularisation_radius,
 	float kinematic_visc);
 
+/* ---- single-source (S2S, S2M) and single-target (M2S) forms ----
+ * Reference libcvtx.h:125-211 (P3D), 266-283 (F3D), 306-360 (P2D); bodies
+ * src/P3D.cpp:74-296, src/F3D.cpp:34-128, src/P2D.cpp:71-231.  Evaluated on
+ * the device through the M2M paths (one device round trip per call).  As
+ * in the reference, the single-source vorticity forms (S2S_vort, S2M_vort,
+ * src/P3D.cpp:146-161, 213-228) have no cut-off and use rho = |r| / sigma,
+ * while M2S_vort cuts at the 5 sigma box like M2M_vort. */
+CVTX_EXPORT bsv_V3f cvtx_P3D_S2S_vel(const cvtx_P3D *self, const bsv_V3f mes_point,
+	const cvtx_VortFunc *kernel, float regularisation_radius);
+CVTX_EXPORT bsv_V3f cvtx_P3D_S2S_dvort(const cvtx_P3D *self, const cvtx_P3D *induced_particle,
+	const cvtx_VortFunc *kernel, float regularisation_radius);
+CVTX_EXPORT bsv_V3f cvtx_P3D_S2S_visc_dvort(const cvtx_P3D *self,
+	const cvtx_P3D *induced_particle, const cvtx_VortFunc *kernel,
+	float regularisation_radius, float kinematic_visc);
+CVTX_EXPORT bsv_V3f cvtx_P3D_S2S_vort(const cvtx_P3D *self, const bsv_V3f mes_point,
+	const cvtx_VortFunc *kernel, float regularisation_radius);
+CVTX_EXPORT void cvtx_P3D_S2M_vel(const cvtx_P3D *self, const bsv_V3f *mes_start,
+	const int num_mes, bsv_V3f *result_array, const cvtx_VortFunc *kernel,
+	float regularisation_radius);
+CVTX_EXPORT void cvtx_P3D_S2M_dvort(const cvtx_P3D *self, const cvtx_P3D **induced_start,
+	const int num_induced, bsv_V3f *result_array, const cvtx_VortFunc *kernel,
+	float regularisation_radius);
+CVTX_EXPORT void cvtx_P3D_S2M_visc_dvort(const cvtx_P3D *self, const cvtx_P3D **induced_start,
+	const int num_induced, bsv_V3f *result_array, const cvtx_VortFunc *kernel,
+	float regularisation_radius, float kinematic_visc);
+CVTX_EXPORT void cvtx_P3D_S2M_vort(const cvtx_P3D *self, const bsv_V3f *mes_start,
+	const int num_mes, bsv_V3f *result_array, const cvtx_VortFunc *kernel,
+	float regularisation_radius);
+CVTX_EXPORT bsv_V3f cvtx_P3D_M2S_vel(const cvtx_P3D **array_start, const int num_particles,
+	const bsv_V3f mes_point, const cvtx_VortFunc *kernel, float regularisation_radius);
+CVTX_EXPORT bsv_V3f cvtx_P3D_M2S_dvort(const cvtx_P3D **array_start, const int num_particles,
+	const cvtx_P3D *induced_particle, const cvtx_VortFunc *kernel,
+	float regularisation_radius);
+CVTX_EXPORT bsv_V3f cvtx_P3D_M2S_visc_dvort(const cvtx_P3D **array_start,
+	const int num_particles, const cvtx_P3D *induced_particle, const cvtx_VortFunc *kernel,
+	float regularisation_radius, float kinematic_visc);
+CVTX_EXPORT bsv_V3f cvtx_P3D_M2S_vort(const cvtx_P3D **array_start, const int num_particles,
+	const bsv_V3f mes_point, const cvtx_VortFunc *kernel, float regularisation_radius);
+
+CVTX_EXPORT bsv_V3f cvtx_F3D_S2S_vel(const cvtx_F3D *self, const bsv_V3f mes_point);
+CVTX_EXPORT bsv_V3f cvtx_F3D_S2S_dvort(const cvtx_F3D *self, const cvtx_P3D *induced_particle);
+CVTX_EXPORT bsv_V3f cvtx_F3D_M2S_vel(const cvtx_F3D **array_start, const int num_filaments,
+	const bsv_V3f mes_point);
+CVTX_EXPORT bsv_V3f cvtx_F3D_M2S_dvort(const cvtx_F3D **array_start, const int num_filaments,
+	const cvtx_P3D *induced_particle);
+
+CVTX_EXPORT bsv_V2f cvtx_P2D_S2S_vel(const cvtx_P2D *self, const bsv_V2f mes_point,
+	const cvtx_VortFunc *kernel, float regularisation_radius);
+CVTX_EXPORT void cvtx_P2D_S2M_vel(const cvtx_P2D *self, const bsv_V2f *mes_start,
+	const int num_mes, bsv_V2f *result_array, const cvtx_VortFunc *kernel,
+	float regularisation_radius);
+CVTX_EXPORT bsv_V2f cvtx_P2D_M2S_vel(const cvtx_P2D **array_start, const int num_particles,
+	const bsv_V2f mes_point, const cvtx_VortFunc *kernel, float regularisation_radius);
+CVTX_EXPORT float cvtx_P2D_S2S_visc_dvort(const cvtx_P2D *self,
+	const cvtx_P2D *induced_particle, const cvtx_VortFunc *kernel,
+	float regularisation_radius, float kinematic_visc);
+CVTX_EXPORT void cvtx_P2D_S2M_visc_dvort(const cvtx_P2D *self, const cvtx_P2D **induced_start,
+	const int num_induced, float *result_array, const cvtx_VortFunc *kernel,
+	float regularisation_radius, float kinematic_visc);
+CVTX_EXPORT float cvtx_P2D_M2S_visc_dvort(const cvtx_P2D **array_start,
+	const int num_particles, const cvtx_P2D *induced_particle, const cvtx_VortFunc *kernel,
+	float regularisation_radius, float kinematic_visc);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
