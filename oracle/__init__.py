@@ -51,6 +51,10 @@ _SIGS = {
     "orc_F3D_M2M_dvort": (None, [_VP, _I, _VP, _I, _VP]),
     "orc_F3D_inf_mtrx": (None, [_VP, _I, _VP, _VP, _I, _VP]),
     "orc64_F3D_inf_mtrx": (None, [_VP, _I, _VP, _VP, _VP, _I, _VP]),
+    "orc_redist_eval": (_F, [_I, _F]),
+    "orc_redist_radius": (_F, [_I]),
+    "orc_P3D_redistribute_on_grid": (_I, [_VP, _I, _VP, _I, _I, _F, _F]),
+    "orc_P2D_redistribute_on_grid": (_I, [_VP, _I, _VP, _I, _I, _F, _F]),
     "orc_P2D_S2S_vel": (V2f, [ctypes.POINTER(P2D), V2f, _I, _F]),
     "orc_P2D_S2S_visc_dvort": (_F, [ctypes.POINTER(P2D), ctypes.POINTER(P2D), _I, _F, _F]),
     "orc_P2D_M2M_vel": (None, [_VP, _I, _VP, _I, _VP, _I, _F]),
@@ -252,6 +256,29 @@ class Oracle:
                                       float(sigma), float(nu))
         return out
 
+    # ---------------- redistribution (remeshing) ----------------
+    def redist(self, rid, U):
+        return self.L.orc_redist_eval(REDIST_ID[rid] if isinstance(rid, str) else rid, float(U))
+
+    def _redistribute(self, fn, particles, dtype, max_out, rid, h, negligible):
+        recs = as_records(particles, dtype)
+        p, keep = pointer_array(recs)
+        r = REDIST_ID[rid] if isinstance(rid, str) else int(rid)
+        n = fn(_vp(p), len(recs), None, 0, r, float(h), float(negligible))
+        cap = n if max_out is None else int(max_out)
+        out = np.zeros(max(cap, 1), dtype)
+        m = fn(_vp(p), len(recs), _vp(out), cap, r, float(h), float(negligible))
+        return out[:m].copy()
+
+    def P3D_redistribute_on_grid(self, particles, rid, h, negligible=0.0, max_out=None):
+        """Remeshed particles (records); max_out None keeps all of them."""
+        return self._redistribute(self.L.orc_P3D_redistribute_on_grid, particles, P3D_DTYPE,
+                                  max_out, rid, h, negligible)
+
+    def P2D_redistribute_on_grid(self, particles, rid, h, negligible=0.0, max_out=None):
+        return self._redistribute(self.L.orc_P2D_redistribute_on_grid, particles, P2D_DTYPE,
+                                  max_out, rid, h, negligible)
+
     # ---------------- FP64 oracle ----------------
     def f64_P3D(self, particles, induced, kernel, sigma, nu=0.0, idx=None, gauss_mode=1,
                 vel=True, dvort=True, visc=False):
@@ -327,6 +354,9 @@ class Oracle:
         self.L.orc64_P2D_M2M_vel_pts(_vp(p), len(p), _vp(mes), _vp(idx), len(idx), _reg(kernel),
                                      float(sigma), gauss_mode, _vp(out))
         return out
+
+
+REDIST_ID = {"lambda0": 0, "lambda1": 1, "lambda2": 2, "lambda3": 3, "m4p": 4}
 
 
 def default():

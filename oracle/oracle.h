@@ -65,6 +65,14 @@ void orc_F3D_M2M_dvort(const cvtx_F3D** src, int nf, const cvtx_P3D** ind, int n
 void orc_F3D_inf_mtrx(const cvtx_F3D** src, int nf, const bsv_V3f* mes, const bsv_V3f* dir,
                       int nm, float* out);
 
+/* redistribution: rid 0..4 = lambda0, lambda1, lambda2, lambda3, m4p */
+float orc_redist_eval(int rid, float U);
+float orc_redist_radius(int rid);
+int orc_P3D_redistribute_on_grid(const cvtx_P3D** in, int n, cvtx_P3D* out, int max_out,
+                                 int rid, float h, float negligible);
+int orc_P2D_redistribute_on_grid(const cvtx_P2D** in, int n, cvtx_P2D* out, int max_out,
+                                 int rid, float h, float negligible);
+
 bsv_V2f orc_P2D_S2S_vel(const cvtx_P2D* self, bsv_V2f mes, int reg, float sigma);
 float orc_P2D_S2S_visc_dvort(const cvtx_P2D* self, const cvtx_P2D* induced, int reg,
                              float sigma, float nu);

@@ -482,6 +482,322 @@ void orc_F3D_inf_mtrx(const cvtx_F3D** src, int nf, const bsv_V3f* mes, const bs
 }
 
 /* ------------------------------------------------------------------ */
+/* Redistribution on a grid (remeshing): reference src/RedistFunc.cpp:  */
+/* 36-96, src/P3D.cpp:509-665, src/P2D.cpp:283-436,                     */
+/* src/redistribution_helper_funcs.cpp:33-80, src/array_methods.cpp.    */
+/* The reference spreads particles over per-thread octrees and merges   */
+/* them; restated here with one thread: contributions are summed per    */
+/* grid node in particle order, and nodes are emitted in the tree's     */
+/* depth-first order, which is ascending Morton order of the node key   */
+/* with the octant index x + 2y + 4z (quadrant x + 2y in 2D).           */
+/* ------------------------------------------------------------------ */
+
+static float rd_lambda0(float U) { return U < 0.5f ? 1.f : 0.f; }
+static float rd_lambda1(float U) { return U <= 1.f ? 1.f - U : 0.f; }
+static float rd_lambda2(float U) {
+	return U < 0.5f ? 1.f - U * U : (U < 1.5f ? 0.5f * (1.f - U) * (2.f - U) : 0.f);
+}
+static float rd_lambda3(float U) {
+	return U < 1.f ? 0.5f * (1.f - U * U) * (2.f - U)
+	               : (U < 2.f ? (1.f / 6.f) * (1.f - U) * (2.f - U) * (3.f - U) : 0.f);
+}
+static float rd_m4p(float U) {
+	return U < 1.f ? 1.f - (ORC_MUT(17) ? 2.0f : 2.5f) * U * U + 1.5f * U * U * U
+	               : (U < 2.f ? 0.5f * (1.f - U) * (2.f - U) * (2.f - U) : 0.f);
+}
+
+static float (*const RD_FUNC[5])(float) = {rd_lambda0, rd_lambda1, rd_lambda2, rd_lambda3, rd_m4p};
+static const float RD_RADIUS[5] = {0.5f, 1.0f, 1.5f, 2.0f, 2.0f};
+
+float orc_redist_eval(int rid, float U) { return RD_FUNC[rid](U); }
+float orc_redist_radius(int rid) { return RD_RADIUS[rid]; }
+
+typedef struct {
+	unsigned long long key; /* Morton code */
+	long seq;               /* particle order, for a stable sort */
+	float s[3];
+} rd_entry;
+
+static int rd_cmp(const void* a, const void* b) {
+	const rd_entry* x = (const rd_entry*)a;
+	const rd_entry* y = (const rd_entry*)b;
+	if (x->key != y->key) return x->key < y->key ? -1 : 1;
+	return x->seq < y->seq ? -1 : (x->seq > y->seq);
+}
+
+static unsigned long long rd_spread3(unsigned v) {
+	unsigned long long r = 0;
+	for (int b = 0; b < 21; ++b) r |= (unsigned long long)((v >> b) & 1u) << (3 * b);
+	return r;
+}
+static unsigned long long rd_spread2(unsigned v) {
+	unsigned long long r = 0;
+	for (int b = 0; b < 32; ++b) r |= (unsigned long long)((v >> b) & 1u) << (2 * b);
+	return r;
+}
+static unsigned rd_compact3(unsigned long long m) {
+	unsigned v = 0;
+	for (int b = 0; b < 21; ++b) v |= (unsigned)((m >> (3 * b)) & 1ull) << b;
+	return v;
+}
+static unsigned rd_compact2(unsigned long long m) {
+	unsigned v = 0;
+	for (int b = 0; b < 32; ++b) v |= (unsigned)((m >> (2 * b)) & 1ull) << b;
+	return v;
+}
+
+/* reference src/redistribution_helper_funcs.cpp:33-80 (n > desired) */
+static float rd_strength_threshold(const float* strs, int n, int n_desired) {
+	double minv, maxv, range;
+	float fminv, fmaxv, tol = 0.01f, guesses[1024];
+	int g_counts[1024], i, k = 0, interp;
+	const int n_guesses = 1024;
+	fminv = fmaxv = n > 0 ? strs[0] : 0.f;
+	for (i = 0; i < n; ++i) {
+		fminv = fminv < strs[i] ? fminv : strs[i];
+		fmaxv = fmaxv > strs[i] ? fmaxv : strs[i];
+	}
+	minv = fminv; maxv = fmaxv;
+	if (n < n_desired) return (float)(maxv * 1.05);
+	while (1) {
+		range = (maxv - minv) * 1.05;
+		for (i = 0; i < n_guesses; ++i) {
+			g_counts[i] = 0;
+			guesses[i] = (float)(minv + i * range / (float)(n_guesses - 1));
+		}
+		for (i = 0; i < n; ++i) {
+			interp = (int)floor((double)(n_guesses - 1) * (strs[i] - minv) / range);
+			if (interp < 0) g_counts[0]++;
+			else if (interp >= n_guesses) g_counts[n_guesses - 1]++;
+			else g_counts[interp]++;
+		}
+		interp = g_counts[n_guesses - 1];
+		for (i = n_guesses - 2; i >= 0; --i) {
+			maxv = guesses[i + 1];
+			minv = guesses[i];
+			interp += g_counts[i];
+			g_counts[i] = interp;
+			if (interp > n_desired) { k = i + 1; break; }
+		}
+		if (minv == maxv || g_counts[k] == g_counts[k - 1] ||
+		    fabs((float)(n_desired - g_counts[k]) / ((float)n_desired)) < tol * 0.6)
+			break;
+	}
+	return guesses[k];
+}
+
+/* reference src/P3D.cpp:638-665 and src/P2D.cpp:410-436: keep particles
+ * stronger than the threshold (and of index < max_keep); the removed
+ * strength is shared equally by the kept ones.  dim = 3 or 1 components
+ * of strength at offset `so` floats into records of `rec` floats. */
+static int rd_remove(float* recs, int rec, int so, int dim, const float* strs, int n,
+                     float thr, int max_keep) {
+	float def[3] = {0.f, 0.f, 0.f};
+	int j = 0;
+	for (int i = 0; i < n; ++i) {
+		float* r = recs + (size_t)i * rec;
+		if (strs[i] > thr && i < max_keep) {
+			if (j != i) memcpy(recs + (size_t)j * rec, r, sizeof(float) * rec);
+			++j;
+		} else if (!ORC_MUT(18)) {
+			for (int c = 0; c < dim; ++c) def[c] = r[so + c] + def[c];
+		}
+	}
+	for (int c = 0; c < dim; ++c) def[c] = def[c] / (float)j;
+	for (int i = 0; i < j; ++i)
+		for (int c = 0; c < dim; ++c) recs[(size_t)i * rec + so + c] = recs[(size_t)i * rec + so + c] + def[c];
+	return j;
+}
+
+static float rd_mean(const float* strs, int n) {
+	float ave = 0.f;
+	for (int i = 0; i < n; ++i) ave += strs[i];
+	return ave / n;
+}
+
+int orc_P3D_redistribute_on_grid(const cvtx_P3D** in, int n, cvtx_P3D* out, int max_out,
+                                 int rid, float h, float negligible) {
+	float (*f)(float) = RD_FUNC[rid];
+	const float rh = 1.f / h;
+	float mn[3] = {0.f, 0.f, 0.f};
+	double sm[3] = {0.0, 0.0, 0.0};
+	if (n > 0) for (int c = 0; c < 3; ++c) mn[c] = in[0]->coord.x[c];
+	for (int i = 0; i < n; ++i)
+		for (int c = 0; c < 3; ++c) {
+			const float x = in[i]->coord.x[c];
+			mn[c] = mn[c] > x ? x : mn[c];
+			sm[c] += (double)x;
+		}
+	float mean[3];
+	for (int c = 0; c < 3; ++c) mean[c] = (float)(sm[c] / (double)n);
+	const int R = (int)roundf(RD_RADIUS[rid]);
+	for (int c = 0; c < 3; ++c) {
+		mn[c] = mn[c] - 1.f * (R * h);
+		float dc = (mean[c] - mn[c]) / h;
+		dc = roundf(dc) + 5;
+		mn[c] = mean[c] - dc * h;
+	}
+	const int w = 2 * R + 1;
+	rd_entry* e = (rd_entry*)malloc(sizeof(rd_entry) * (size_t)(n > 0 ? n : 1) * w * w * w);
+	size_t ne = 0;
+	for (long i = 0; i < n; ++i) {
+		const bsv_V3f p = in[i]->coord;
+		const bsv_V3f a = in[i]->vorticity;
+		unsigned kx = (unsigned)roundf((p.x[0] - mn[0]) * rh);
+		unsigned ky = (unsigned)roundf((p.x[1] - mn[1]) * rh);
+		unsigned kz = (unsigned)roundf((p.x[2] - mn[2]) * rh);
+		for (int ii = -R; ii <= R; ++ii)
+			for (int jj = -R; jj <= R; ++jj)
+				for (int kk = -R; kk <= R; ++kk) {
+					const unsigned qx = kx + ii, qy = ky + jj, qz = kz + kk;
+					const float nx = mn[0] + h * (float)qx, ny = mn[1] + h * (float)qy,
+					            nz = mn[2] + h * (float)qz;
+					const float U = fabsf((p.x[0] - nx) * rh);
+					const float W = fabsf((p.x[1] - ny) * rh);
+					const float V = fabsf((p.x[2] - nz) * rh);
+					const float frac = f(U) * f(W) * f(V);
+					const float s0 = a.x[0] * frac, s1 = a.x[1] * frac, s2 = a.x[2] * frac;
+					if (s0 == 0.f && s1 == 0.f && s2 == 0.f) continue;
+					rd_entry* t = &e[ne++];
+					t->key = rd_spread3(qx) | (rd_spread3(qy) << 1) | (rd_spread3(qz) << 2);
+					t->seq = (long)ne;
+					t->s[0] = s0; t->s[1] = s1; t->s[2] = s2;
+				}
+	}
+	qsort(e, ne, sizeof(rd_entry), rd_cmp);
+	/* nodes: {x, y, z, ax, ay, az, vol} as float records */
+	float* nodes = (float*)malloc(sizeof(float) * 7 * (ne > 0 ? ne : 1));
+	int nn = 0;
+	for (size_t i = 0; i < ne; ++i) {
+		if (i == 0 || e[i].key != e[i - 1].key) {
+			const unsigned qx = rd_compact3(e[i].key), qy = rd_compact3(e[i].key >> 1),
+			               qz = rd_compact3(e[i].key >> 2);
+			float* r = nodes + 7 * (size_t)nn++;
+			r[0] = mn[0] + h * (float)qx;
+			r[1] = mn[1] + h * (float)qy;
+			r[2] = mn[2] + h * (float)qz;
+			r[3] = r[4] = r[5] = 0.f;
+			r[6] = h * h * h;
+		}
+		float* r = nodes + 7 * (size_t)(nn - 1);
+		for (int c = 0; c < 3; ++c) r[3 + c] = r[3 + c] + e[i].s[c];
+	}
+	free(e);
+	float* strs = (float*)malloc(sizeof(float) * (nn > 0 ? nn : 1));
+	for (int i = 0; i < nn; ++i) {
+		const float* r = nodes + 7 * (size_t)i;
+		strs[i] = sqrtf(r[3] * r[3] + r[4] * r[4] + r[5] * r[5]);
+	}
+	float thr = rd_mean(strs, nn) * negligible;
+	nn = rd_remove(nodes, 7, 3, 3, strs, nn, thr, nn);
+	for (int i = 0; i < nn; ++i) {
+		const float* r = nodes + 7 * (size_t)i;
+		strs[i] = sqrtf(r[3] * r[3] + r[4] * r[4] + r[5] * r[5]);
+	}
+	if (out != NULL) {
+		if (nn > max_out) {
+			thr = rd_strength_threshold(strs, nn, max_out);
+			nn = rd_remove(nodes, 7, 3, 3, strs, nn, thr, max_out);
+		}
+		for (int i = 0; i < nn; ++i) {
+			const float* r = nodes + 7 * (size_t)i;
+			for (int c = 0; c < 3; ++c) {
+				out[i].coord.x[c] = r[c];
+				out[i].vorticity.x[c] = r[3 + c];
+			}
+			out[i].volume = r[6];
+		}
+	}
+	free(strs);
+	free(nodes);
+	return nn;
+}
+
+int orc_P2D_redistribute_on_grid(const cvtx_P2D** in, int n, cvtx_P2D* out, int max_out,
+                                 int rid, float h, float negligible) {
+	float (*f)(float) = RD_FUNC[rid];
+	const float rh = 1.f / h;
+	float mn[2] = {0.f, 0.f};
+	double sm[2] = {0.0, 0.0};
+	if (n > 0) for (int c = 0; c < 2; ++c) mn[c] = in[0]->coord.x[c];
+	for (int i = 0; i < n; ++i)
+		for (int c = 0; c < 2; ++c) {
+			const float x = in[i]->coord.x[c];
+			mn[c] = mn[c] > x ? x : mn[c];
+			sm[c] += (double)x;
+		}
+	float mean[2];
+	for (int c = 0; c < 2; ++c) mean[c] = (float)(sm[c] / (double)n);
+	const int R = (int)roundf(RD_RADIUS[rid]);
+	for (int c = 0; c < 2; ++c) {
+		mn[c] = mn[c] - 1.f * (R * h);
+		float dc = (mean[c] - mn[c]) / h;
+		dc = roundf(dc) + 5;
+		mn[c] = mean[c] - dc * h;
+	}
+	const int w = 2 * R + 1;
+	rd_entry* e = (rd_entry*)malloc(sizeof(rd_entry) * (size_t)(n > 0 ? n : 1) * w * w);
+	size_t ne = 0;
+	for (long i = 0; i < n; ++i) {
+		const bsv_V2f p = in[i]->coord;
+		const float a = in[i]->vorticity;
+		/* reference UIntKey64.cpp:79-96: the offset is formed in double */
+		unsigned kx = (unsigned)roundf((float)(((double)p.x[0] - mn[0]) * rh));
+		unsigned ky = (unsigned)roundf((float)(((double)p.x[1] - mn[1]) * rh));
+		for (int ii = -R; ii <= R; ++ii)
+			for (int jj = -R; jj <= R; ++jj) {
+				const unsigned qx = kx + ii, qy = ky + jj;
+				const float nx = mn[0] + h * (float)qx, ny = mn[1] + h * (float)qy;
+				const float U = fabsf((p.x[0] - nx) * rh);
+				const float W = fabsf((p.x[1] - ny) * rh);
+				const float s0 = a * (f(U) * f(W));
+				if (s0 == 0.f) continue;
+				rd_entry* t = &e[ne++];
+				t->key = rd_spread2(qx) | (rd_spread2(qy) << 1);
+				t->seq = (long)ne;
+				t->s[0] = s0;
+			}
+	}
+	qsort(e, ne, sizeof(rd_entry), rd_cmp);
+	/* nodes: {x, y, a, area} */
+	float* nodes = (float*)malloc(sizeof(float) * 4 * (ne > 0 ? ne : 1));
+	int nn = 0;
+	for (size_t i = 0; i < ne; ++i) {
+		if (i == 0 || e[i].key != e[i - 1].key) {
+			float* r = nodes + 4 * (size_t)nn++;
+			r[0] = mn[0] + h * (float)rd_compact2(e[i].key);
+			r[1] = mn[1] + h * (float)rd_compact2(e[i].key >> 1);
+			r[2] = 0.f;
+			r[3] = h * h;
+		}
+		float* r = nodes + 4 * (size_t)(nn - 1);
+		r[2] = r[2] + e[i].s[0];
+	}
+	free(e);
+	float* strs = (float*)malloc(sizeof(float) * (nn > 0 ? nn : 1));
+	for (int i = 0; i < nn; ++i) strs[i] = fabsf(nodes[4 * (size_t)i + 2]);
+	float thr = rd_mean(strs, nn) * negligible;
+	nn = rd_remove(nodes, 4, 2, 1, strs, nn, thr, nn);
+	for (int i = 0; i < nn; ++i) strs[i] = fabsf(nodes[4 * (size_t)i + 2]);
+	if (out != NULL) {
+		if (nn > max_out) {
+			thr = rd_strength_threshold(strs, nn, max_out);
+			nn = rd_remove(nodes, 4, 2, 1, strs, nn, thr, max_out);
+		}
+		for (int i = 0; i < nn; ++i) {
+			const float* r = nodes + 4 * (size_t)i;
+			out[i].coord.x[0] = r[0];
+			out[i].coord.x[1] = r[1];
+			out[i].vorticity = r[2];
+			out[i].area = r[3];
+		}
+	}
+	free(strs);
+	free(nodes);
+	return nn;
+}
+
+/* ------------------------------------------------------------------ */
 /* P2D: reference src/P2D.cpp:49-276                                   */
 /* ------------------------------------------------------------------ */
 

@@ -565,7 +565,80 @@ def influence_matrix_rows(o):
     return f
 
 
+# ---------------------------------------------------------------------------
+# K. redistribution on a grid (remeshing), reference src/RedistFunc.cpp:
+#    36-96, src/P3D.cpp:509-665, src/P2D.cpp:283-436.  The interpolation
+#    kernels form a partition of unity (total strength is conserved); all
+#    but lambda0 reproduce linear functions (first moments conserved) and
+#    lambda2, lambda3, m4p also quadratics (second moments conserved).
+#    Strength removed as negligible, or to meet the output limit, is shared
+#    among the kept particles, so the total is conserved in every case.
+# ---------------------------------------------------------------------------
+RD = ("lambda0", "lambda1", "lambda2", "lambda3", "m4p")
+
+
+def _moments3(recs):
+    x = recs["coord"].astype(np.float64)
+    a = recs["vorticity"].astype(np.float64)
+    return a.sum(0), x.T @ a, (x * x).T @ a
+
+
+def redistribution_moments(o):
+    f = []
+    cl = W.random_p3d(60, 29, box=1.0, vort=1.0)
+    cl["vorticity"] -= 0.5
+    h = 0.1
+    m0, m1, m2 = _moments3(cl)
+    for r in RD:
+        out = o.P3D_redistribute_on_grid(cl, r, h)
+        n0, n1, n2 = _moments3(out)
+        if np.abs(n0 - m0).max() > 1e-5 * np.abs(cl["vorticity"]).sum():
+            f.append(f"{r}: total strength not conserved")
+        if r != "lambda0" and np.abs(n1 - m1).max() > 1e-4 * np.abs(m1).max():
+            f.append(f"{r}: first moment not conserved")
+        if r in ("lambda2", "lambda3", "m4p") and np.abs(n2 - m2).max() > 1e-4 * np.abs(m2).max():
+            f.append(f"{r}: second moment not conserved")
+        if np.any(np.abs(out["volume"] - h ** 3) > 1e-9):
+            f.append(f"{r}: new particles have volume h^3")
+        # removal of the weakest particles keeps the total
+        cut = o.P3D_redistribute_on_grid(cl, r, h, negligible=0.3)
+        if len(cut) > len(out) or np.abs(_moments3(cut)[0] - m0).max() > 1e-5 * np.abs(cl["vorticity"]).sum():
+            f.append(f"{r}: negligible-strength removal must conserve the total")
+        # the limit keeps, among the first max_out particles, those above a
+        # strength threshold chosen for about max_out survivors overall
+        # (src/P3D.cpp:622-629, 645-651): at most max_out remain
+        lim = o.P3D_redistribute_on_grid(cl, r, h, max_out=len(out) // 2)
+        if not 0 < len(lim) <= len(out) // 2 or \
+                np.abs(_moments3(lim)[0] - m0).max() > 1e-5 * np.abs(cl["vorticity"]).sum():
+            f.append(f"{r}: output limit must hold, conserving the total")
+    # one particle: with the weak spill-over removed, one particle remains
+    one = make_p3d([[0.3, -0.2, 0.7]], [[1.0, -2.0, 0.5]], [0.001])
+    for r in RD:
+        out = o.P3D_redistribute_on_grid(one, r, 0.05, negligible=0.5)
+        if len(out) != 1 or np.abs(out["coord"][0] - one["coord"][0]).max() > 1e-6 or \
+                np.abs(out["vorticity"][0] - one["vorticity"][0]).max() > 1e-5:
+            f.append(f"{r}: a particle on a grid node stays itself")
+    # 2D: the same conservation laws
+    q = W.random_p2d(80, 30, box=1.0, vort=1.0)
+    q["vorticity"] -= 0.5
+    x = q["coord"].astype(np.float64)
+    a = q["vorticity"].astype(np.float64)
+    for r in RD:
+        out = o.P2D_redistribute_on_grid(q, r, 0.1)
+        y = out["coord"].astype(np.float64)
+        b = out["vorticity"].astype(np.float64)
+        if abs(b.sum() - a.sum()) > 1e-5 * np.abs(a).sum():
+            f.append(f"{r}: 2D total not conserved")
+        if r != "lambda0" and np.abs(y.T @ b - x.T @ a).max() > 1e-4 * np.abs(x.T @ a).max():
+            f.append(f"{r}: 2D first moment not conserved")
+        cut = o.P2D_redistribute_on_grid(q, r, 0.1, negligible=0.3)
+        if abs(cut["vorticity"].astype(np.float64).sum() - a.sum()) > 1e-5 * np.abs(a).sum():
+            f.append(f"{r}: 2D removal must conserve the total")
+    return f
+
+
 ALL_CHECKS = (kat_vortfunc, moments_3d, g_cumulative_zeta, kernels_2d, particle_invariants,
               biot_savart_pair, dvort_transpose_fd, pse_conservation, f3d_vel_quadrature,
               f3d_dvort_structure, p2d_point_vortex, ring_centre, refcpu_vs_fp64,
-              vort_box_cutoff, relaxation_invariants, influence_matrix_rows)
+              vort_box_cutoff, relaxation_invariants, influence_matrix_rows,
+              redistribution_moments)

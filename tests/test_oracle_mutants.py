@@ -34,6 +34,8 @@ EXPECTED = {
     14: "vort_box_cutoff",      # vorticity cut-off box -> sphere
     15: "relaxation_invariants",  # relaxation keeps fdt alpha instead of (1 - fdt) alpha
     16: "influence_matrix_rows",  # influence matrix stored column-major
+    17: "redistribution_moments",  # m4p inner coefficient 2.5 -> 2.0
+    18: "redistribution_moments",  # removed strength not redistributed
 }
 
 
