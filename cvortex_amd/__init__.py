@@ -18,8 +18,8 @@ import os
 
 import numpy as np
 
-from .types import (F3D_DTYPE, P2D_DTYPE, P3D_DTYPE, REG_ID, REG_NAMES, V2f, V3f, VortFunc,
-                    as_records, pointer_array)
+from .types import (F3D_DTYPE, P2D_DTYPE, P3D_DTYPE, REG_ID, REG_NAMES, RedistFunc, V2f, V3f,
+                    VortFunc, as_records, pointer_array)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcvortex.so")
 
@@ -55,6 +55,13 @@ _SIGNATURES = {
     "cvtx_VortFunc_gaussian": (VortFunc, []),
     # public M2M (void)
     "cvtx_P3D_M2M_vel": (None, [_VP, _I, _VP, _I, _VP, _VP, _F]),
+    "cvtx_RedistFunc_lambda0": (RedistFunc, []),
+    "cvtx_RedistFunc_lambda1": (RedistFunc, []),
+    "cvtx_RedistFunc_lambda2": (RedistFunc, []),
+    "cvtx_RedistFunc_lambda3": (RedistFunc, []),
+    "cvtx_RedistFunc_m4p": (RedistFunc, []),
+    "cvtx_P3D_redistribute_on_grid": (_I, [_VP, _I, _VP, _I, _VP, _F, _F]),
+    "cvtx_P2D_redistribute_on_grid": (_I, [_VP, _I, _VP, _I, _VP, _F, _F]),
     # single-source / single-target forms (records by pointer, points by value)
     "cvtx_P3D_S2S_vel": (V3f, [_VP, V3f, _VP, _F]),
     "cvtx_P3D_S2S_dvort": (V3f, [_VP, _VP, _VP, _F]),
@@ -577,3 +584,42 @@ def P2D_M2S_visc_dvort(particles, ind, kernel, sigma, nu):
     ri, k2 = _rec(ind, P2D_DTYPE)
     k, kp = _kref(kernel)
     return float(lib().cvtx_P2D_M2S_visc_dvort(_vp(p), len(p), ri, kp, float(sigma), float(nu)))
+
+
+# ------------------- redistribution on a grid (remeshing) -------------------
+
+REDIST_NAMES = ("lambda0", "lambda1", "lambda2", "lambda3", "m4p")
+
+
+def redistfunc(name):
+    """The library's cvtx_RedistFunc for a kernel name (libcvtx.h:118-123)."""
+    if name not in REDIST_NAMES:
+        raise ValueError(f"unknown redistribution kernel {name!r}")
+    return getattr(lib(), f"cvtx_RedistFunc_{name}")()
+
+
+def _redistribute(fn, particles, dtype, redistributor, grid_density, negligible, max_out):
+    p, keep = _ptrs(particles, dtype)
+    r = redistfunc(redistributor) if isinstance(redistributor, str) else redistributor
+    rp = ctypes.cast(ctypes.pointer(r), ctypes.c_void_p)
+    n = fn(_vp(p), len(p), None, 0, rp, float(grid_density), float(negligible))
+    if n < 0:
+        raise CvtxError(f"redistribute_on_grid: {last_error()}")
+    cap = n if max_out is None else int(max_out)
+    out = np.zeros(max(cap, 1), dtype)
+    m = fn(_vp(p), len(p), _vp(out), cap, rp, float(grid_density), float(negligible))
+    if m < 0:
+        raise CvtxError(f"redistribute_on_grid: {last_error()}")
+    return out[:m].copy()
+
+
+def P3D_redistribute_on_grid(particles, redistributor, grid_density, negligible=0.0, max_out=None):
+    """Remeshed P3D records (libcvtx.h:250-257); max_out None keeps all."""
+    return _redistribute(lib().cvtx_P3D_redistribute_on_grid, particles, P3D_DTYPE, redistributor,
+                         grid_density, negligible, max_out)
+
+
+def P2D_redistribute_on_grid(particles, redistributor, grid_density, negligible=0.0, max_out=None):
+    """Remeshed P2D records (libcvtx.h:372-380); max_out None keeps all."""
+    return _redistribute(lib().cvtx_P2D_redistribute_on_grid, particles, P2D_DTYPE, redistributor,
+                         grid_density, negligible, max_out)

@@ -36,6 +36,11 @@ int* order_points(Runtime& rt, Device& d, const float4* pts, int n, int slot, hi
 // false for the reference's single-source forms (no 5 sigma box).
 int vort_host(const cvtx_P3D** src, int ns, const bsv_V3f* mes, int nt, float* out,
               const cvtx_VortFunc* vf, float sigma, bool cutoff);
+// Redistribution on a grid (redist.hip): returns the number of output
+// particles (>= 0) or -1 with the error recorded.  With j.max_out >= 0,
+// *out_dev holds the packed records (7 floats per P3D, 4 per P2D) in a
+// library buffer, valid until the next call.
+int redistribute(Runtime& rt, Device& d, const RedistJob& j, float** out_dev, hipStream_t s);
 // explicit launch shape: T in {1, 2, 4}, n_split >= 1 source partitions
 // (rounded to whole 256-source tiles); false for an invalid shape
 bool make_launch_shape(int T, int n_split, int n_src, LaunchCfg* c);

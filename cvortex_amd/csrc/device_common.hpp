@@ -41,6 +41,24 @@ struct RegConsts {
   float c2;        // gaussian: sqrt(2/pi)
 };
 
+// ---- separately rounded FP32 operations --------------------------------
+// For code that must reproduce another program's FP32 operation sequence:
+// each product and sum is rounded on its own.  (HIP's __fmul_rn family is
+// plain arithmetic, which floating-point contraction may fuse into a
+// multiply-add with a single rounding.)
+__device__ __forceinline__ float rn_mul(float a, float b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ float rn_add(float a, float b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+__device__ __forceinline__ float rn_sub(float a, float b) {
+#pragma clang fp contract(off)
+  return a - b;
+}
+
 // ---- compensated (TwoSum) accumulation ----------------------------------
 // Knuth's TwoSum: hi + lo carries the running sum exactly up to the final
 // rounding of lo.  Six FP32 operations per update; applied once per source

@@ -94,6 +94,21 @@ struct VortJob {
   bool cutoff = true;              // false: no 5 sigma box (single-source forms)
 };
 
+// Redistribution on a grid (remeshing).  dim 3: pv = {x, y, z, volume},
+// a = {alpha, -}; dim 2: pv = {x, y, alpha, area}, a unused.  rid 0..4 =
+// lambda0, lambda1, lambda2, lambda3, m4'.  max_out < 0: count only.
+struct RedistJob {
+  int dim = 3;
+  int rid = 4;
+  const float4* pv = nullptr;
+  const float4* a = nullptr;
+  int n = 0;
+  float h = 1.f;
+  float negligible = 0.f;
+  int max_out = -1;
+};
+float redist_radius(int rid);
+
 // number of FP64 partial accumulators per target for a mask
 int p3d_nacc(int mask);
 int p2d_nacc(int mask);

@@ -1,0 +1,662 @@
+// redist.hip -- redistribution of vortex particles onto a regular grid
+// (remeshing): cvtx_P3D_redistribute_on_grid, cvtx_P2D_redistribute_on_grid.
+//
+// Semantics of the reference CPU path (src/P3D.cpp:509-665,
+// src/P2D.cpp:283-436, src/RedistFunc.cpp:36-96,
+// src/redistribution_helper_funcs.cpp:33-80):
+//   1. grid origin from the particles' minimum and mean (one cell layout for
+//      the whole call), node key = round((x - origin) / h) per axis;
+//   2. every particle gives strength alpha * f(U) f(W) (f(V)) to the
+//      (2R+1)^dim nodes around its nearest node, R = round(radius), with
+//      U = |x - x_node| / h per axis; exactly-zero contributions make no node;
+//   3. the nodes' summed strengths become particles of volume h^dim, in the
+//      ascending Morton order of the node keys (octant index x + 2y + 4z,
+//      quadrant x + 2y), the order the reference's tree emits them in;
+//   4. particles weaker than negligible x mean strength are removed and their
+//      strength shared equally by the rest;
+//   5. with an output array and more particles than it holds, a histogram
+//      search picks a strength threshold for about max_output survivors, and
+//      of the first max_output particles those above it are kept, again
+//      sharing the removed strength.
+// The weights use the reference's FP32 operations in its order (no fused
+// multiply-adds), so the set of nodes, their positions and the weights of
+// every contribution are the reference's; node sums are formed in FP64 in
+// particle order (the reference sums in FP32).
+//
+// GPU structure: one thread per (particle, neighbour) writes a (Morton key,
+// contribution) entry; rocPRIM's stable radix sort groups the entries by
+// node; a thread per node sums its run; removal and truncation are
+// order-preserving compactions (rocPRIM select) with the removed strength
+// reduced in FP64.  Scalar control (grid origin, thresholds, the histogram
+// search's loop) runs on the host between kernels.
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+namespace cvtx {
+
+namespace {
+
+constexpr int kPartBlocks = 512;  // fixed grid of the reductions (deterministic)
+constexpr unsigned long long kSentinel = ~0ull;
+
+// Redistribution functions, reference src/RedistFunc.cpp:36-96, in FP32
+// with the reference's operation order.
+template <int RID>
+__device__ __forceinline__ float rd_f(float U) {
+  if constexpr (RID == 0) {  // lambda0
+    return U < 0.5f ? 1.f : 0.f;
+  } else if constexpr (RID == 1) {  // lambda1
+    return U <= 1.f ? rn_sub(1.f, U) : 0.f;
+  } else if constexpr (RID == 2) {  // lambda2
+    return U < 0.5f ? rn_sub(1.f, rn_mul(U, U))
+                    : (U < 1.5f ? rn_mul(rn_mul(0.5f, rn_sub(1.f, U)), rn_sub(2.f, U))
+                                : 0.f);
+  } else if constexpr (RID == 3) {  // lambda3
+    return U < 1.f ? rn_mul(rn_mul(0.5f, rn_sub(1.f, rn_mul(U, U))), rn_sub(2.f, U))
+                   : (U < 2.f ? rn_mul(rn_mul(rn_mul(1.f / 6.f, rn_sub(1.f, U)),
+                                                    rn_sub(2.f, U)),
+                                          rn_sub(3.f, U))
+                              : 0.f);
+  } else {  // m4'
+    return U < 1.f ? rn_add(rn_sub(1.f, rn_mul(rn_mul(2.5f, U), U)),
+                               rn_mul(rn_mul(rn_mul(1.5f, U), U), U))
+                   : (U < 2.f ? rn_mul(rn_mul(rn_mul(0.5f, rn_sub(1.f, U)),
+                                                    rn_sub(2.f, U)),
+                                          rn_sub(2.f, U))
+                              : 0.f);
+  }
+}
+
+__host__ __device__ __forceinline__ unsigned long long spread3(unsigned v) {
+  unsigned long long x = v & 0x1fffffull;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+__host__ __device__ __forceinline__ unsigned compact3(unsigned long long x) {
+  x &= 0x1249249249249249ull;
+  x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
+  x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
+  x = (x ^ (x >> 8)) & 0x1f0000ff0000ffull;
+  x = (x ^ (x >> 16)) & 0x1f00000000ffffull;
+  x = (x ^ (x >> 32)) & 0x1fffffull;
+  return (unsigned)x;
+}
+__host__ __device__ __forceinline__ unsigned long long spread2(unsigned v) {
+  unsigned long long x = v;
+  x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+  x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+  x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+  x = (x | (x << 2)) & 0x3333333333333333ull;
+  x = (x | (x << 1)) & 0x5555555555555555ull;
+  return x;
+}
+__host__ __device__ __forceinline__ unsigned compact2(unsigned long long x) {
+  x &= 0x5555555555555555ull;
+  x = (x ^ (x >> 1)) & 0x3333333333333333ull;
+  x = (x ^ (x >> 2)) & 0x0f0f0f0f0f0f0f0full;
+  x = (x ^ (x >> 4)) & 0x00ff00ff00ff00ffull;
+  x = (x ^ (x >> 8)) & 0x0000ffff0000ffffull;
+  x = (x ^ (x >> 16)) & 0x00000000ffffffffull;
+  return (unsigned)x;
+}
+
+struct Grid {
+  float mn[3];  // origin
+  float h, rh;  // spacing and its reciprocal (FP32 as the reference)
+  int R;        // neighbourhood half-width
+};
+
+// Per-block min (FP32) and sum (FP64) of each coordinate.
+template <int DIM>
+__global__ __launch_bounds__(256) void rd_bounds(const float4* __restrict__ pv, int n,
+                                                 float* __restrict__ pmin,
+                                                 float* __restrict__ pmax,
+                                                 double* __restrict__ psum) {
+  __shared__ float smin[DIM][256];
+  __shared__ float smax[DIM][256];
+  __shared__ double ssum[DIM][256];
+  float m[DIM], M[DIM];
+  double s[DIM];
+  for (int c = 0; c < DIM; ++c) { m[c] = __builtin_inff(); M[c] = -__builtin_inff(); s[c] = 0.0; }
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const float4 p = pv[i];
+    const float x[3] = {p.x, p.y, p.z};
+    for (int c = 0; c < DIM; ++c) {
+      m[c] = fminf(m[c], x[c]);
+      M[c] = fmaxf(M[c], x[c]);
+      s[c] += (double)x[c];
+    }
+  }
+  for (int c = 0; c < DIM; ++c) {
+    smin[c][threadIdx.x] = m[c];
+    smax[c][threadIdx.x] = M[c];
+    ssum[c][threadIdx.x] = s[c];
+  }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int c = 0; c < DIM; ++c) {
+        smin[c][threadIdx.x] = fminf(smin[c][threadIdx.x], smin[c][threadIdx.x + w]);
+        smax[c][threadIdx.x] = fmaxf(smax[c][threadIdx.x], smax[c][threadIdx.x + w]);
+        ssum[c][threadIdx.x] += ssum[c][threadIdx.x + w];
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int c = 0; c < DIM; ++c) {
+      pmin[blockIdx.x * DIM + c] = smin[c][0];
+      pmax[blockIdx.x * DIM + c] = smax[c][0];
+      psum[blockIdx.x * DIM + c] = ssum[c][0];
+    }
+}
+
+// One thread per (particle, neighbour): entry e = i * W^DIM + j.
+// 3D: pv = {x, y, z, -}, a = {alpha, -}; 2D: pv = {x, y, alpha, -}.
+template <int DIM, int RID>
+__global__ __launch_bounds__(256) void rd_contrib(const float4* __restrict__ pv,
+                                                  const float4* __restrict__ a, long n_entries,
+                                                  Grid g, unsigned long long* __restrict__ keys,
+                                                  float4* __restrict__ vals) {
+  // n_entries < 2^31 (checked on the host): 32-bit indices
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= n_entries) return;
+  const int W = 2 * g.R + 1;
+  const int WD = DIM == 3 ? W * W * W : W * W;
+  const int i = e / WD;
+  const int j = e - i * WD;
+  const float4 p = pv[i];
+  const float x[3] = {p.x, p.y, p.z};
+  unsigned q[3];
+  int off[3];
+  if constexpr (DIM == 3) {
+    off[0] = j / (W * W) - g.R;
+    off[1] = (j / W) % W - g.R;
+    off[2] = j % W - g.R;
+  } else {
+    off[0] = j / W - g.R;
+    off[1] = j % W - g.R;
+  }
+  float frac = 1.f;
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) {
+    float k;
+    if constexpr (DIM == 3) {  // reference UIntKey96.cpp: FP32 offset
+      k = roundf(rn_mul(rn_sub(x[c], g.mn[c]), g.rh));
+    } else {  // reference UIntKey64.cpp:79-96: the offset is formed in FP64
+      k = roundf((float)(((double)x[c] - (double)g.mn[c]) * (double)g.rh));
+    }
+    q[c] = (unsigned)k + (unsigned)off[c];
+    const float node = rn_add(g.mn[c], rn_mul(g.h, (float)q[c]));
+    const float U = fabsf(rn_mul(rn_sub(x[c], node), g.rh));
+    frac = c == 0 ? rd_f<RID>(U) : rn_mul(frac, rd_f<RID>(U));
+  }
+  float4 v;
+  unsigned long long key;
+  if constexpr (DIM == 3) {
+    const float4 al = a[i];
+    v = make_float4(rn_mul(al.x, frac), rn_mul(al.y, frac), rn_mul(al.z, frac), 0.f);
+    const bool zero = v.x == 0.f && v.y == 0.f && v.z == 0.f;
+    key = zero ? kSentinel : (spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2));
+  } else {
+    v = make_float4(rn_mul(p.z, frac), 0.f, 0.f, 0.f);
+    key = v.x == 0.f ? kSentinel : (spread2(q[0]) | (spread2(q[1]) << 1));
+  }
+  keys[e] = key;
+  vals[e] = v;
+}
+
+__global__ __launch_bounds__(256) void rd_heads(const unsigned long long* __restrict__ keys,
+                                                long n, unsigned char* __restrict__ head) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long k = keys[i];
+  head[i] = (k != kSentinel && (i == 0 || keys[i - 1] != k)) ? 1 : 0;
+}
+
+// One thread per node: sum its run of entries (FP64, entry order), place it.
+// pos = {x, y, z, volume} (2D: {x, y, -, area}); w = strength; str = |w|.
+template <int DIM>
+__global__ __launch_bounds__(256) void rd_nodes(const unsigned long long* __restrict__ keys,
+                                                const float4* __restrict__ vals, long n_entries,
+                                                const long* __restrict__ starts, int n_nodes,
+                                                Grid g, float4* __restrict__ pos,
+                                                float4* __restrict__ w, float* __restrict__ str) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n_nodes) return;
+  const long s = starts[k];
+  const unsigned long long key = keys[s];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (long i = s; i < n_entries && keys[i] == key; ++i) {
+    const float4 v = vals[i];
+    a0 += (double)v.x;
+    a1 += (double)v.y;
+    a2 += (double)v.z;
+  }
+  const float vol = DIM == 3 ? rn_mul(rn_mul(g.h, g.h), g.h) : rn_mul(g.h, g.h);
+  float4 p;
+  if constexpr (DIM == 3) {
+    p = make_float4(rn_add(g.mn[0], rn_mul(g.h, (float)compact3(key))),
+                    rn_add(g.mn[1], rn_mul(g.h, (float)compact3(key >> 1))),
+                    rn_add(g.mn[2], rn_mul(g.h, (float)compact3(key >> 2))), vol);
+  } else {
+    p = make_float4(rn_add(g.mn[0], rn_mul(g.h, (float)compact2(key))),
+                    rn_add(g.mn[1], rn_mul(g.h, (float)compact2(key >> 1))), 0.f, vol);
+  }
+  const float4 ww = make_float4((float)a0, (float)a1, (float)a2, 0.f);
+  pos[k] = p;
+  w[k] = ww;
+  str[k] = DIM == 3 ? __fsqrt_rn(rn_add(rn_add(rn_mul(ww.x, ww.x), rn_mul(ww.y, ww.y)),
+                                           rn_mul(ww.z, ww.z)))
+                    : fabsf(ww.x);
+}
+
+// Per-block FP64 sum and FP32 min / max of the strengths.
+__global__ __launch_bounds__(256) void rd_str_stats(const float* __restrict__ str, int n,
+                                                    double* __restrict__ psum,
+                                                    float* __restrict__ pmin,
+                                                    float* __restrict__ pmax) {
+  __shared__ double ss[256];
+  __shared__ float sl[256], sh[256];
+  double s = 0.0;
+  float lo = __builtin_inff(), hi = -__builtin_inff();
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const float v = str[i];
+    s += (double)v;
+    lo = fminf(lo, v);
+    hi = fmaxf(hi, v);
+  }
+  ss[threadIdx.x] = s;
+  sl[threadIdx.x] = lo;
+  sh[threadIdx.x] = hi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      ss[threadIdx.x] += ss[threadIdx.x + w];
+      sl[threadIdx.x] = fminf(sl[threadIdx.x], sl[threadIdx.x + w]);
+      sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    psum[blockIdx.x] = ss[0];
+    pmin[blockIdx.x] = sl[0];
+    pmax[blockIdx.x] = sh[0];
+  }
+}
+
+// keep[i] = str[i] > thr && i < max_keep; per-block FP64 sums of the
+// removed strength (3 components).
+__global__ __launch_bounds__(256) void rd_flags(const float* __restrict__ str,
+                                                const float4* __restrict__ w, int n, float thr,
+                                                int max_keep, unsigned char* __restrict__ keep,
+                                                double* __restrict__ pdef) {
+  __shared__ double sd[3][256];
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const bool k = str[i] > thr && i < max_keep;
+    keep[i] = k ? 1 : 0;
+    if (!k) {
+      const float4 v = w[i];
+      d0 += (double)v.x;
+      d1 += (double)v.y;
+      d2 += (double)v.z;
+    }
+  }
+  sd[0][threadIdx.x] = d0;
+  sd[1][threadIdx.x] = d1;
+  sd[2][threadIdx.x] = d2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int c = 0; c < 3; ++c) sd[c][threadIdx.x] += sd[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 3; ++c) pdef[blockIdx.x * 3 + c] = sd[c][0];
+}
+
+// Gather the kept particles, add the shared deficit, refresh |w|.
+template <int DIM>
+__global__ __launch_bounds__(256) void rd_keep(const int* __restrict__ idx, int n_kept,
+                                               const float4* __restrict__ pos_in,
+                                               const float4* __restrict__ w_in, float4 def,
+                                               float4* __restrict__ pos_out,
+                                               float4* __restrict__ w_out,
+                                               float* __restrict__ str_out) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n_kept) return;
+  const int i = idx[j];
+  pos_out[j] = pos_in[i];
+  const float4 v = w_in[i];
+  const float4 ww = make_float4(rn_add(v.x, def.x), rn_add(v.y, def.y),
+                                rn_add(v.z, def.z), 0.f);
+  w_out[j] = ww;
+  str_out[j] = DIM == 3 ? __fsqrt_rn(rn_add(rn_add(rn_mul(ww.x, ww.x), rn_mul(ww.y, ww.y)),
+                                               rn_mul(ww.z, ww.z)))
+                        : fabsf(ww.x);
+}
+
+// Histogram of the strength-threshold search (reference
+// src/redistribution_helper_funcs.cpp:54-60): bin = floor(1023 (s - minv) / range).
+__global__ __launch_bounds__(256) void rd_hist(const float* __restrict__ str, int n, double minv,
+                                               double range, int* __restrict__ counts) {
+  __shared__ int sc[1024];
+  for (int b = threadIdx.x; b < 1024; b += 256) sc[b] = 0;
+  __syncthreads();
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    int b = (int)floor((double)1023 * ((double)str[i] - minv) / range);
+    b = b < 0 ? 0 : (b > 1023 ? 1023 : b);
+    atomicAdd(&sc[b], 1);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < 1024; b += 256)
+    if (sc[b]) atomicAdd(&counts[b], sc[b]);
+}
+
+// AoS output records: 3D {coord[3], vorticity[3], volume} (7 floats),
+// 2D {coord[2], vorticity, area} (4 floats).
+template <int DIM>
+__global__ __launch_bounds__(256) void rd_pack(const float4* __restrict__ pos,
+                                               const float4* __restrict__ w, int n,
+                                               float* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = pos[i];
+  const float4 v = w[i];
+  if constexpr (DIM == 3) {
+    float* r = out + 7 * (size_t)i;
+    r[0] = p.x; r[1] = p.y; r[2] = p.z; r[3] = v.x; r[4] = v.y; r[5] = v.z; r[6] = p.w;
+  } else {
+    float* r = out + 4 * (size_t)i;
+    r[0] = p.x; r[1] = p.y; r[2] = v.x; r[3] = p.w;
+  }
+}
+
+size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Bump allocator over one runtime workspace slot.
+struct Arena {
+  char* base = nullptr;
+  size_t used = 0, cap = 0;
+  template <class T>
+  T* take(size_t n) {
+    T* p = (T*)(base + used);
+    used += al256(sizeof(T) * (n > 0 ? n : 1));
+    return used <= cap ? p : nullptr;
+  }
+};
+
+template <int DIM>
+hipError_t launch_contrib(int rid, const float4* pv, const float4* a, long ne, const Grid& g,
+                          unsigned long long* keys, float4* vals, hipStream_t s) {
+  const dim3 grid((unsigned)((ne + 255) / 256));
+  switch (rid) {
+    case 0: hipLaunchKernelGGL((rd_contrib<DIM, 0>), grid, dim3(256), 0, s, pv, a, ne, g, keys, vals); break;
+    case 1: hipLaunchKernelGGL((rd_contrib<DIM, 1>), grid, dim3(256), 0, s, pv, a, ne, g, keys, vals); break;
+    case 2: hipLaunchKernelGGL((rd_contrib<DIM, 2>), grid, dim3(256), 0, s, pv, a, ne, g, keys, vals); break;
+    case 3: hipLaunchKernelGGL((rd_contrib<DIM, 3>), grid, dim3(256), 0, s, pv, a, ne, g, keys, vals); break;
+    default: hipLaunchKernelGGL((rd_contrib<DIM, 4>), grid, dim3(256), 0, s, pv, a, ne, g, keys, vals); break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+float redist_radius(int rid) {
+  static const float r[5] = {0.5f, 1.0f, 1.5f, 2.0f, 2.0f};
+  return (rid >= 0 && rid < 5) ? r[rid] : -1.f;
+}
+
+template <int DIM>
+static int redistribute_dim(Runtime& rt, Device& d, const RedistJob& j, float** out_dev,
+                            hipStream_t s) {
+  auto fail = [&](const char* what, hipError_t e) {
+    if (e != hipSuccess) check_hip(rt, e, what);
+    else set_error(rt, what);
+    return -1;
+  };
+  const int n = j.n;
+  *out_dev = nullptr;
+  if (n == 0) return 0;
+  // ---- grid origin (reference src/P3D.cpp:538-548) ----
+  float* pmin = nullptr;
+  float* pmax = nullptr;
+  double* psum = nullptr;
+  {
+    Arena small;
+    const size_t bytes = 2 * al256(sizeof(float) * kPartBlocks * 3) + al256(sizeof(double) * kPartBlocks * 3);
+    small.base = (char*)device_buffer(rt, d, SLOT_RD_SMALL, bytes);
+    small.cap = bytes;
+    if (!small.base) return -1;
+    pmin = small.take<float>(kPartBlocks * 3);
+    pmax = small.take<float>(kPartBlocks * 3);
+    psum = small.take<double>(kPartBlocks * 3);
+  }
+  hipLaunchKernelGGL((rd_bounds<DIM>), dim3(kPartBlocks), dim3(256), 0, s, j.pv, n, pmin, pmax, psum);
+  std::vector<float> hmin(kPartBlocks * DIM), hmax(kPartBlocks * DIM);
+  std::vector<double> hsum(kPartBlocks * DIM);
+  hipError_t e = hipMemcpyAsync(hmin.data(), pmin, sizeof(float) * hmin.size(), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(hmax.data(), pmax, sizeof(float) * hmax.size(), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(hsum.data(), psum, sizeof(double) * hsum.size(), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail("redistribution bounds", e);
+  Grid g{};
+  g.h = j.h;
+  g.rh = 1.f / j.h;
+  g.R = (int)roundf(redist_radius(j.rid));
+  for (int c = 0; c < DIM; ++c) {
+    float m = hmin[c], M = hmax[c];
+    double sum = 0.0;
+    for (int b = 0; b < kPartBlocks; ++b) {
+      m = hmin[b * DIM + c] < m ? hmin[b * DIM + c] : m;
+      M = hmax[b * DIM + c] > M ? hmax[b * DIM + c] : M;
+      sum += hsum[b * DIM + c];
+    }
+    if (!std::isfinite(m) || !std::isfinite(M)) return fail("non-finite particle coordinates", hipSuccess);
+    const float mean = (float)(sum / (double)n);
+    float mn = m - 1.f * ((float)g.R * j.h);
+    float dc = (mean - mn) / j.h;
+    dc = roundf(dc) + 5;
+    g.mn[c] = mean - dc * j.h;
+    // node indices must fit the Morton key: 21 bits per axis in 3D, 32 in 2D
+    const double span = ((double)M - (double)g.mn[c]) / (double)j.h + g.R + 2;
+    if (span >= (DIM == 3 ? 2097152.0 : 4294967295.0))
+      return fail("grid too fine for the particles' extent", hipSuccess);
+  }
+  const int W = 2 * g.R + 1;
+  const long WD = DIM == 3 ? (long)W * W * W : (long)W * W;
+  const long ne = (long)n * WD;
+  if (ne >= 2147483647L) return fail("too many grid contributions for one call", hipSuccess);
+  // ---- workspace ----
+  size_t sort_tmp = 0, sel_tmp = 0;
+  e = rocprim::radix_sort_pairs(nullptr, sort_tmp, (unsigned long long*)nullptr,
+                                (unsigned long long*)nullptr, (float4*)nullptr, (float4*)nullptr,
+                                (size_t)ne, 0, 64, s);
+  if (e != hipSuccess) return fail("radix sort size", e);
+  e = rocprim::select(nullptr, sel_tmp, rocprim::counting_iterator<long>(0),
+                      (unsigned char*)nullptr, (long*)nullptr, (long*)nullptr, (size_t)ne, s);
+  if (e != hipSuccess) return fail("select size", e);
+  size_t sel_tmp2 = 0;
+  e = rocprim::select(nullptr, sel_tmp2, rocprim::counting_iterator<int>(0), (unsigned char*)nullptr,
+                      (int*)nullptr, (int*)nullptr, (size_t)ne, s);
+  if (e != hipSuccess) return fail("select size", e);
+  const size_t tmp = std::max(sort_tmp, std::max(sel_tmp, sel_tmp2));
+  const size_t need = al256(sizeof(unsigned long long) * ne) * 2 + al256(sizeof(float4) * ne) * 2 +
+                      al256(tmp) + al256(ne) + al256(sizeof(long) * ne) + al256(sizeof(long)) +
+                      al256(sizeof(float4) * ne) * 4 + al256(sizeof(float) * ne) * 2 +
+                      al256(sizeof(int) * ne) + al256(sizeof(int)) + al256(sizeof(double) * kPartBlocks * 3) +
+                      al256(sizeof(float) * kPartBlocks) * 2 + al256(sizeof(int) * 1024) +
+                      al256(sizeof(float) * 7 * ne);
+  Arena ar;
+  ar.base = (char*)device_buffer(rt, d, SLOT_RD, need);
+  ar.cap = need;
+  if (!ar.base) return -1;
+  auto* k_in = ar.take<unsigned long long>(ne);
+  auto* k_out = ar.take<unsigned long long>(ne);
+  auto* v_in = ar.take<float4>(ne);
+  auto* v_out = ar.take<float4>(ne);
+  void* t = ar.take<char>(tmp);
+  auto* head = ar.take<unsigned char>(ne);
+  auto* starts = ar.take<long>(ne);
+  auto* nstart = ar.take<long>(1);
+  auto* pos0 = ar.take<float4>(ne);
+  auto* w0 = ar.take<float4>(ne);
+  auto* pos1 = ar.take<float4>(ne);
+  auto* w1 = ar.take<float4>(ne);
+  auto* str0 = ar.take<float>(ne);
+  auto* str1 = ar.take<float>(ne);
+  auto* kidx = ar.take<int>(ne);
+  auto* nkept = ar.take<int>(1);
+  auto* pdef = ar.take<double>(kPartBlocks * 3);
+  auto* psmin = ar.take<float>(kPartBlocks);
+  auto* psmax = ar.take<float>(kPartBlocks);
+  auto* counts = ar.take<int>(1024);
+  auto* packed = ar.take<float>(7 * (size_t)ne);
+  if (!packed) return fail("redistribution workspace", hipSuccess);
+  // ---- contributions, sort, node sums ----
+  e = launch_contrib<DIM>(j.rid, j.pv, j.a, ne, g, k_in, v_in, s);
+  if (e != hipSuccess) return fail("redistribution contributions", e);
+  size_t tb = tmp;
+  e = rocprim::radix_sort_pairs(t, tb, k_in, k_out, v_in, v_out, (size_t)ne, 0, 64, s);
+  if (e != hipSuccess) return fail("redistribution sort", e);
+  hipLaunchKernelGGL(rd_heads, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, k_out, ne, head);
+  tb = tmp;
+  e = rocprim::select(t, tb, rocprim::counting_iterator<long>(0), head, starts, nstart, (size_t)ne, s);
+  if (e != hipSuccess) return fail("redistribution node starts", e);
+  long hn = 0;
+  e = hipMemcpyAsync(&hn, nstart, sizeof(long), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail("redistribution node count", e);
+  int nn = (int)hn;
+  if (nn == 0) return 0;
+  hipLaunchKernelGGL((rd_nodes<DIM>), dim3((nn + 255) / 256), dim3(256), 0, s, k_out, v_out, ne,
+                     starts, nn, g, pos0, w0, str0);
+  // ---- removal: strength <= threshold and index >= max_keep ----
+  auto remove = [&](float thr, int max_keep) -> int {
+    hipLaunchKernelGGL(rd_flags, dim3(kPartBlocks), dim3(256), 0, s, str0, w0, nn, thr, max_keep,
+                       head, pdef);
+    size_t tb2 = tmp;
+    hipError_t ee = rocprim::select(t, tb2, rocprim::counting_iterator<int>(0), head, kidx, nkept,
+                                    (size_t)nn, s);
+    if (ee != hipSuccess) return fail("redistribution compaction", ee);
+    std::vector<double> hdef(kPartBlocks * 3);
+    int hk = 0;
+    ee = hipMemcpyAsync(hdef.data(), pdef, sizeof(double) * hdef.size(), hipMemcpyDeviceToHost, s);
+    if (ee == hipSuccess) ee = hipMemcpyAsync(&hk, nkept, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (ee == hipSuccess) ee = hipStreamSynchronize(s);
+    if (ee != hipSuccess) return fail("redistribution compaction", ee);
+    double dd[3] = {0.0, 0.0, 0.0};
+    for (int b = 0; b < kPartBlocks; ++b)
+      for (int c = 0; c < 3; ++c) dd[c] += hdef[b * 3 + c];
+    // the reference divides the removed strength by the kept count in FP32
+    const float4 def = make_float4((float)dd[0] / (float)hk, (float)dd[1] / (float)hk,
+                                   (float)dd[2] / (float)hk, 0.f);
+    if (hk > 0)
+      hipLaunchKernelGGL((rd_keep<DIM>), dim3((hk + 255) / 256), dim3(256), 0, s, kidx, hk, pos0,
+                         w0, def, pos1, w1, str1);
+    std::swap(pos0, pos1);
+    std::swap(w0, w1);
+    std::swap(str0, str1);
+    nn = hk;
+    return 0;
+  };
+  auto stats = [&](double* sum, float* lo, float* hi) -> bool {
+    hipLaunchKernelGGL(rd_str_stats, dim3(kPartBlocks), dim3(256), 0, s, str0, nn, pdef, psmin, psmax);
+    std::vector<double> hs(kPartBlocks);
+    std::vector<float> hl(kPartBlocks), hh(kPartBlocks);
+    hipError_t ee = hipMemcpyAsync(hs.data(), pdef, sizeof(double) * kPartBlocks, hipMemcpyDeviceToHost, s);
+    if (ee == hipSuccess) ee = hipMemcpyAsync(hl.data(), psmin, sizeof(float) * kPartBlocks, hipMemcpyDeviceToHost, s);
+    if (ee == hipSuccess) ee = hipMemcpyAsync(hh.data(), psmax, sizeof(float) * kPartBlocks, hipMemcpyDeviceToHost, s);
+    if (ee == hipSuccess) ee = hipStreamSynchronize(s);
+    if (ee != hipSuccess) {
+      fail("redistribution strengths", ee);
+      return false;
+    }
+    double a = 0.0;
+    float l = __builtin_inff(), u = -__builtin_inff();
+    for (int b = 0; b < kPartBlocks; ++b) {
+      a += hs[b];
+      l = hl[b] < l ? hl[b] : l;
+      u = hh[b] > u ? hh[b] : u;
+    }
+    *sum = a;
+    *lo = l;
+    *hi = u;
+    return true;
+  };
+  double ssum;
+  float smin, smax;
+  if (!stats(&ssum, &smin, &smax)) return -1;
+  const float mean = (float)ssum / (float)nn;
+  if (remove(mean * j.negligible, nn) != 0) return -1;
+  if (j.max_out >= 0 && nn > j.max_out) {
+    // strength threshold, reference src/redistribution_helper_funcs.cpp:33-80
+    if (!stats(&ssum, &smin, &smax)) return -1;
+    double minv = smin, maxv = smax, range;
+    const int n_guesses = 1024;
+    std::vector<float> guesses(n_guesses);
+    std::vector<int> gc(n_guesses);
+    int k = 0;
+    const float tol = 0.01f;
+    for (int iter = 0; iter < 4096; ++iter) {
+      range = (maxv - minv) * 1.05;
+      for (int i = 0; i < n_guesses; ++i)
+        guesses[i] = (float)(minv + i * range / (float)(n_guesses - 1));
+      e = hipMemsetAsync(counts, 0, sizeof(int) * 1024, s);
+      if (e != hipSuccess) return fail("redistribution histogram", e);
+      hipLaunchKernelGGL(rd_hist, dim3(kPartBlocks), dim3(256), 0, s, str0, nn, minv, range, counts);
+      e = hipMemcpyAsync(gc.data(), counts, sizeof(int) * 1024, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return fail("redistribution histogram", e);
+      int interp = gc[n_guesses - 1];
+      for (int i = n_guesses - 2; i >= 0; --i) {
+        maxv = guesses[i + 1];
+        minv = guesses[i];
+        interp += gc[i];
+        gc[i] = interp;
+        if (interp > j.max_out) {
+          k = i + 1;
+          break;
+        }
+      }
+      if (minv == maxv || gc[k] == gc[k - 1] ||
+          std::fabs((float)(j.max_out - gc[k]) / ((float)j.max_out)) < tol * 0.6)
+        break;
+    }
+    if (remove(guesses[k], j.max_out) != 0) return -1;
+  }
+  if (j.max_out < 0 || nn == 0) return nn;
+  hipLaunchKernelGGL((rd_pack<DIM>), dim3((nn + 255) / 256), dim3(256), 0, s, pos0, w0, nn, packed);
+  e = hipGetLastError();
+  if (e != hipSuccess) return fail("redistribution output", e);
+  *out_dev = packed;
+  return nn;
+}
+
+int redistribute(Runtime& rt, Device& d, const RedistJob& j, float** out_dev, hipStream_t s) {
+  if (j.rid < 0 || j.rid > 4 || !(j.h > 0.f) || j.n < 0 || (j.n > 0 && !j.pv) ||
+      (j.dim == 3 && j.n > 0 && !j.a)) {
+    set_error(rt, "invalid redistribution job");
+    return -1;
+  }
+  return j.dim == 3 ? redistribute_dim<3>(rt, d, j, out_dev, s)
+                    : redistribute_dim<2>(rt, d, j, out_dev, s);
+}
+
+}  // namespace cvtx

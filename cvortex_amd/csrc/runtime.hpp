@@ -31,6 +31,8 @@ enum Slot : int {
   SLOT_ORDER,  // target-order workspace
   SLOT_PERM,   // target permutation
   SLOT_PERM2,  // source permutation (vorticity field)
+  SLOT_RD,     // redistribution workspace
+  SLOT_RD_SMALL,
   SLOT_COUNT
 };
 

@@ -65,9 +65,9 @@ struct Zeta<REG_PLANETARY> {  // 3 inside the core, 0 outside -> 1 / 0
   // (src/P3D.cpp:154-155).
   static __device__ __forceinline__ float eval(float dx, float dy, float dz, float rsigma,
                                                float sigma, bool divide, const RegConsts&) {
-    const float r2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+    const float r2 = rn_add(rn_add(rn_mul(dx, dx), rn_mul(dy, dy)), rn_mul(dz, dz));
     const float r = __fsqrt_rn(r2);
-    const float rho = divide ? __fdiv_rn(r, sigma) : __fmul_rn(r, rsigma);
+    const float rho = divide ? __fdiv_rn(r, sigma) : rn_mul(r, rsigma);
     return rho < 1.f ? 1.f : 0.f;
   }
 };
@@ -313,16 +313,16 @@ __global__ __launch_bounds__(256) void relax_kernel(float4* __restrict__ a,
   const float4 ov = a[i];
   const float wx = omega[3 * i + 0], wy = omega[3 * i + 1], wz = omega[3 * i + 2];
   const float aw =
-      __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(wx, wx), __fmul_rn(wy, wy)), __fmul_rn(wz, wz)));
+      __fsqrt_rn(rn_add(rn_add(rn_mul(wx, wx), rn_mul(wy, wy)), rn_mul(wz, wz)));
   const float aa = __fsqrt_rn(
-      __fadd_rn(__fadd_rn(__fmul_rn(ov.x, ov.x), __fmul_rn(ov.y, ov.y)), __fmul_rn(ov.z, ov.z)));
+      rn_add(rn_add(rn_mul(ov.x, ov.x), rn_mul(ov.y, ov.y)), rn_mul(ov.z, ov.z)));
   const float coeff = __fdiv_rn(aa, aw);
-  const float tmp = __fsub_rn(1.f, fdt);
-  const float cf = __fmul_rn(coeff, fdt);
+  const float tmp = rn_sub(1.f, fdt);
+  const float cf = rn_mul(coeff, fdt);
   float4 nv;
-  nv.x = __fadd_rn(__fmul_rn(ov.x, tmp), __fmul_rn(wx, cf));
-  nv.y = __fadd_rn(__fmul_rn(ov.y, tmp), __fmul_rn(wy, cf));
-  nv.z = __fadd_rn(__fmul_rn(ov.z, tmp), __fmul_rn(wz, cf));
+  nv.x = rn_add(rn_mul(ov.x, tmp), rn_mul(wx, cf));
+  nv.y = rn_add(rn_mul(ov.y, tmp), rn_mul(wy, cf));
+  nv.z = rn_add(rn_mul(ov.z, tmp), rn_mul(wz, cf));
   nv.w = ov.w;
   if (aw == 0.f) nv = make_float4(0.f, 0.f, 0.f, ov.w);
   a[i] = nv;

@@ -32,6 +32,11 @@ class V2f(ctypes.Structure):
     _fields_ = [("x", ctypes.c_float * 2)]
 
 
+class RedistFunc(ctypes.Structure):
+    """cvtx_RedistFunc: interpolation kernel plugin (func pointer + radius)."""
+    _fields_ = [("func", ctypes.c_void_p), ("radius", ctypes.c_float)]
+
+
 class P3D(ctypes.Structure):
     _fields_ = [("coord", V3f), ("vorticity", V3f), ("volume", ctypes.c_float)]
 

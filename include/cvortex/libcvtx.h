@@ -208,6 +208,45 @@ CVTX_EXPORT void cvtx_P2D_M2M_visc_dvort(
 	float regularisation_radius,
 	float kinematic_visc);
 
+/* ---- redistribution on a grid (remeshing) ----
+ * Interpolation kernels, reference libcvtx.h:96-99, 118-123,
+ * src/RedistFunc.cpp:36-96.  The device recognises the library's own five
+ * functions; a cvtx_RedistFunc around another function is rejected. */
+typedef struct {
+	float(*func)(float U);
+	float radius;
+} cvtx_RedistFunc;
+
+CVTX_EXPORT const cvtx_RedistFunc cvtx_RedistFunc_lambda0(void);
+CVTX_EXPORT const cvtx_RedistFunc cvtx_RedistFunc_lambda1(void);
+CVTX_EXPORT const cvtx_RedistFunc cvtx_RedistFunc_lambda2(void);
+CVTX_EXPORT const cvtx_RedistFunc cvtx_RedistFunc_lambda3(void);
+CVTX_EXPORT const cvtx_RedistFunc cvtx_RedistFunc_m4p(void);
+
+/* Redistribute particles onto a grid of spacing grid_density; particles
+ * weaker than negligible_vort x the mean strength are removed (their
+ * strength shared among the rest).  With output_particles NULL returns the
+ * number of particles created; otherwise writes at most
+ * max_output_particles of them (in grid Morton order) and returns the
+ * number written.  -1 on failure (message on stderr).  Reference
+ * libcvtx.h:250-257 and 372-380, src/P3D.cpp:509-665, src/P2D.cpp:283-436. */
+CVTX_EXPORT int cvtx_P3D_redistribute_on_grid(
+	const cvtx_P3D **input_array_start,
+	const int n_input_particles,
+	cvtx_P3D *output_particles,
+	int max_output_particles,
+	const cvtx_RedistFunc *redistributor,
+	float grid_density,
+	float negligible_vort);
+CVTX_EXPORT int cvtx_P2D_redistribute_on_grid(
+	const cvtx_P2D **input_array_start,
+	const int n_input_particles,
+	cvtx_P2D *output_particles,
+	int max_output_particles,
+	const cvtx_RedistFunc *redistributor,
+	float grid_density,
+	float negligible_vort);
+
 /* ---- single-source (S2S, S2M) and single-target (M2S) forms ----
  * Reference libcvtx.h:125-211 (P3D), 266-283 (F3D), 306-360 (P2D); bodies
  * src/P3D.cpp:74-296, src/F3D.cpp:34-128, src/P2D.cpp:71-231.  Evaluated on

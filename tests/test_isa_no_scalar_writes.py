@@ -19,7 +19,7 @@ FORBIDDEN = re.compile(r"^\s*s_(store|atomic|buffer_store|buffer_atomic|dcache_w
 
 @pytest.mark.skipif(shutil.which("hipcc") is None and not os.path.exists("/opt/rocm/bin/hipcc"),
                     reason="hipcc not available")
-@pytest.mark.parametrize("name", ["p3d", "p2d", "f3d", "batched", "order"])
+@pytest.mark.parametrize("name", ["p3d", "p2d", "f3d", "batched", "order", "vort", "redist"])
 def test_no_scalar_memory_writes(tmp_path, name):
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     out = tmp_path / f"{name}.s"

@@ -171,3 +171,27 @@ def test_no_device_public_api_reports_nan(L):
     finally:
         for i, st in enumerate(states):
             (cv.accelerator_enable if st else cv.accelerator_disable)(i)
+
+
+def test_redistribution_functions_match_oracle(L, orc):
+    # the plugin's host functions (reference src/RedistFunc.cpp:36-96)
+    fn_t = ctypes.CFUNCTYPE(ctypes.c_float, ctypes.c_float)
+    radii = {"lambda0": 0.5, "lambda1": 1.0, "lambda2": 1.5, "lambda3": 2.0, "m4p": 2.0}
+    for k, name in enumerate(cv.REDIST_NAMES):
+        r = cv.redistfunc(name)
+        assert r.radius == radii[name]
+        f = fn_t(r.func)
+        for U in np.linspace(0.0, 2.6, 105).astype(np.float32):
+            assert f(float(U)) == orc.redist(k, float(U)), (name, U)
+
+
+def test_foreign_redistributor_is_rejected(L):
+    # a cvtx_RedistFunc around a function the library does not own has no
+    # device kernel: -1 and a message (the reference would evaluate it)
+    from cvortex_amd.types import RedistFunc
+    fn_t = ctypes.CFUNCTYPE(ctypes.c_float, ctypes.c_float)
+    own = fn_t(lambda U: 1.0 if U < 0.5 else 0.0)
+    r = RedistFunc(ctypes.cast(own, ctypes.c_void_p), 0.5)
+    p = np.zeros(3, P3D_DTYPE)
+    with pytest.raises(cv.CvtxError, match="redistributor"):
+        cv.P3D_redistribute_on_grid(p, r, 0.1)
