@@ -227,7 +227,9 @@ __global__ __launch_bounds__(256) void rd_heads(const unsigned long long* __rest
   head[i] = (k != kSentinel && (i == 0 || keys[i - 1] != k)) ? 1 : 0;
 }
 
-// One thread per node: sum its run of entries (FP64, entry order), place it.
+// One wave per node: the wave reads the node's run of entries 64 at a time
+// (coalesced) and reduces it in FP64 with a fixed shuffle tree, so the sum
+// is the same on every call; the run ends at the first different key.
 // pos = {x, y, z, volume} (2D: {x, y, -, area}); w = strength; str = |w|.
 template <int DIM>
 __global__ __launch_bounds__(256) void rd_nodes(const unsigned long long* __restrict__ keys,
@@ -235,17 +237,30 @@ __global__ __launch_bounds__(256) void rd_nodes(const unsigned long long* __rest
                                                 const long* __restrict__ starts, int n_nodes,
                                                 Grid g, float4* __restrict__ pos,
                                                 float4* __restrict__ w, float* __restrict__ str) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= n_nodes) return;
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= n_nodes) return;  // whole waves leave together
   const long s = starts[k];
   const unsigned long long key = keys[s];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  for (long i = s; i < n_entries && keys[i] == key; ++i) {
-    const float4 v = vals[i];
-    a0 += (double)v.x;
-    a1 += (double)v.y;
-    a2 += (double)v.z;
+  for (long base = s;; base += 64) {
+    const long i = base + lane;
+    const bool in = i < n_entries && keys[i] == key;
+    if (in) {
+      const float4 v = vals[i];
+      a0 += (double)v.x;
+      a1 += (double)v.y;
+      a2 += (double)v.z;
+    }
+    // the run is contiguous: it continues only if every lane was inside it
+    if (__ballot(in) != ~0ull) break;
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    a0 += __shfl_xor(a0, o);
+    a1 += __shfl_xor(a1, o);
+    a2 += __shfl_xor(a2, o);
+  }
+  if (lane != 0) return;
   const float vol = DIM == 3 ? rn_mul(rn_mul(g.h, g.h), g.h) : rn_mul(g.h, g.h);
   float4 p;
   if constexpr (DIM == 3) {
@@ -260,7 +275,7 @@ __global__ __launch_bounds__(256) void rd_nodes(const unsigned long long* __rest
   pos[k] = p;
   w[k] = ww;
   str[k] = DIM == 3 ? __fsqrt_rn(rn_add(rn_add(rn_mul(ww.x, ww.x), rn_mul(ww.y, ww.y)),
-                                           rn_mul(ww.z, ww.z)))
+                                        rn_mul(ww.z, ww.z)))
                     : fabsf(ww.x);
 }
 
@@ -460,6 +475,7 @@ static int redistribute_dim(Runtime& rt, Device& d, const RedistJob& j, float** 
   g.h = j.h;
   g.rh = 1.f / j.h;
   g.R = (int)roundf(redist_radius(j.rid));
+  int axis_bits = 1;  // bits of the largest node index on any axis
   for (int c = 0; c < DIM; ++c) {
     float m = hmin[c], M = hmax[c];
     double sum = 0.0;
@@ -478,7 +494,11 @@ static int redistribute_dim(Runtime& rt, Device& d, const RedistJob& j, float** 
     const double span = ((double)M - (double)g.mn[c]) / (double)j.h + g.R + 2;
     if (span >= (DIM == 3 ? 2097152.0 : 4294967295.0))
       return fail("grid too fine for the particles' extent", hipSuccess);
+    while (axis_bits < 32 && (double)(1ull << axis_bits) <= span) ++axis_bits;
   }
+  // the sort only needs the interleaved bits of valid keys plus one more,
+  // above which every valid key is zero and the sentinel is not
+  const int key_bits = DIM * axis_bits + 1 < 64 ? DIM * axis_bits + 1 : 64;
   const int W = 2 * g.R + 1;
   const long WD = DIM == 3 ? (long)W * W * W : (long)W * W;
   const long ne = (long)n * WD;
@@ -533,7 +553,7 @@ static int redistribute_dim(Runtime& rt, Device& d, const RedistJob& j, float** 
   e = launch_contrib<DIM>(j.rid, j.pv, j.a, ne, g, k_in, v_in, s);
   if (e != hipSuccess) return fail("redistribution contributions", e);
   size_t tb = tmp;
-  e = rocprim::radix_sort_pairs(t, tb, k_in, k_out, v_in, v_out, (size_t)ne, 0, 64, s);
+  e = rocprim::radix_sort_pairs(t, tb, k_in, k_out, v_in, v_out, (size_t)ne, 0, key_bits, s);
   if (e != hipSuccess) return fail("redistribution sort", e);
   hipLaunchKernelGGL(rd_heads, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, k_out, ne, head);
   tb = tmp;
@@ -545,7 +565,7 @@ static int redistribute_dim(Runtime& rt, Device& d, const RedistJob& j, float** 
   if (e != hipSuccess) return fail("redistribution node count", e);
   int nn = (int)hn;
   if (nn == 0) return 0;
-  hipLaunchKernelGGL((rd_nodes<DIM>), dim3((nn + 255) / 256), dim3(256), 0, s, k_out, v_out, ne,
+  hipLaunchKernelGGL((rd_nodes<DIM>), dim3((nn + 3) / 4), dim3(256), 0, s, k_out, v_out, ne,
                      starts, nn, g, pos0, w0, str0);
   // ---- removal: strength <= threshold and index >= max_keep ----
   auto remove = [&](float thr, int max_keep) -> int {
