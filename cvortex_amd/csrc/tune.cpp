@@ -66,7 +66,20 @@ int size_bucket(int n) {
 LaunchCfg choose_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt, int n_cu) {
   auto it = cache().find(Key{kind, reg, mask, acc, size_bucket(n_src), size_bucket(n_tgt)});
   if (it != cache().end()) return shape(it->second.T, it->second.n_split, n_src);
-  return default_cfg(n_src, n_tgt, n_cu);
+  LaunchCfg c = default_cfg(n_src, n_tgt, n_cu);
+  // Spatially ordered targets (Gaussian P3D, compensated sums): a
+  // 256-target workgroup's waves are more compact than a 512-target one's,
+  // so fewer of their pairs take the FP64 near-field branch (2^20 ring:
+  // vel+dvort 723.0 ms against 726.8 ms, +visc 747 ms against 786 ms).  The
+  // plain FP32 kernel has no such branch and loses its second target's
+  // reuse instead (861 ms against 678 ms), so it keeps T = 2.  Keep T = 1
+  // only when it still gives several workgroups per CU without source
+  // partitions.
+  if (kind == KIND_P3D && acc == ACC_COMP && p3d_wants_order(reg, n_src, n_tgt) && c.T == 2 &&
+      c.n_split == 1 &&
+      (long long)(n_tgt + 255) / 256 >= 4LL * (n_cu > 0 ? n_cu : 256))
+    c.T = 1;
+  return c;
 }
 
 void record_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt, int T, int n_split) {
