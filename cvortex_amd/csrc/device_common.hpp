@@ -39,7 +39,37 @@ struct RegConsts {
   float inv_s2;    // 1/sigma^2
   float c1;        // gaussian: -log2(e)/2 | 2D winckelmans eta: 4 log2(e)
   float c2;        // gaussian: sqrt(2/pi)
+  float q_far = 208.f;  // gaussian: rho^2 beyond which exp(-rho^2/2) is 0 in FP32
+  float q_tab = (float)(GTAB_RHO_MAX * GTAB_RHO_MAX);  // gaussian: table range in rho^2
 };
+
+// A uniform value that a pair loop reads on every pair, held in a VGPR.  On
+// gfx950 a VALU instruction with an SGPR operand issues at about half the
+// rate of one whose operands are VGPRs, inline constants or a literal
+// (measured: 1.8 issue slots against 1), and the compiler keeps uniform
+// values in SGPRs.  The empty asm makes the value opaque, so it stays in a
+// VGPR for the loop.
+__device__ __forceinline__ float in_vgpr(float x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+__device__ __forceinline__ int in_vgpr(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+// The constants of `k` as VGPR values (see in_vgpr), for use inside pair loops.
+__device__ __forceinline__ RegConsts loop_consts(const RegConsts& k) {
+  RegConsts v;
+  v.s2 = in_vgpr(k.s2);
+  v.inv_s2 = in_vgpr(k.inv_s2);
+  v.c1 = in_vgpr(k.c1);
+  v.c2 = in_vgpr(k.c2);
+  v.q_far = in_vgpr(k.q_far);
+  v.q_tab = in_vgpr(k.q_tab);
+  return v;
+}
 
 // ---- separately rounded FP32 operations --------------------------------
 // For code that must reproduce another program's FP32 operation sequence:
@@ -160,36 +190,53 @@ struct RegKernel<REG_GAUSSIAN> {
   // called inside the rho^2 < 208 branch (only there is eta nonzero), and
   // Kfar is Kv for the pairs beyond it and 0 for the others.  A caller can so
   // accumulate the near pairs in a wider type at the cost of the branch only.
+  // The evaluation in two steps, for callers that interleave pairs:
+  // far() is the rho^2 >= 208 form every pair starts from, near() replaces
+  // g and h and sets zeta for a pair with rho^2 < 208 (the caller tests
+  // q < k.q_far).  eval_ranges is the two combined.
+  struct Far {
+    float q, pinv, pinv2, g, h;
+  };
+  static __device__ __forceinline__ Far far(float r2, const RegConsts& k) {
+    Far f;
+    f.q = __builtin_fmaf(r2, k.inv_s2, 1e-30f);  // rho^2; the 1e-30 keeps rho finite at r2 = 0
+    f.pinv = __builtin_amdgcn_rsqf(f.q);         // 1/rho
+    f.pinv2 = f.pinv * f.pinv;
+    f.g = f.pinv2 * f.pinv;
+    f.h = f.g * f.pinv2;
+    return f;
+  }
+  static __device__ __forceinline__ void near(const Far& f, float r2, const RegConsts& k,
+                                              const float4* tab, float& g, float& h,
+                                              float& z) {
+    z = k.c2 * __builtin_amdgcn_exp2f(f.q * k.c1);
+    g = f.pinv2 * (f.pinv - z);
+    h = __builtin_fmaf(z, -1.f / 3.f, g) * f.pinv2;
+    if (f.q < k.q_tab) {
+      float t = f.q * f.pinv * (float)GTAB_PER_UNIT;  // rho * 16
+      int i = (int)t;
+      i = i < (GTAB_N - 1) ? i : (GTAB_N - 1);
+      i = i > 0 ? i : 0;
+      float u = t - (float)i;
+      float4 cg = tab[2 * i];
+      float4 ch = tab[2 * i + 1];
+      g = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(cg.w, u, cg.z), u, cg.y), u, cg.x);
+      h = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(ch.w, u, ch.z), u, ch.y), u, ch.x);
+      bool zr = (r2 == 0.f);
+      g = zr ? 0.f : g;
+      z = zr ? 0.f : z;
+    }
+  }
+
   template <class Near>
   static __device__ __forceinline__ void eval_ranges(float r2, const RegConsts& k,
                                                      const float4* tab, float& Kv, float& Kh,
-                                                     float& eta, float& Kfar, Near&& near) {
-    float q = __builtin_fmaf(r2, k.inv_s2, 1e-30f);  // rho^2; the 1e-30 keeps rho finite at r2 = 0
-    float pinv = __builtin_amdgcn_rsqf(q);           // 1/rho
-    float pinv2 = pinv * pinv;
-    float g = pinv2 * pinv;
-    float h = g * pinv2;
-    float z = 0.f;
-    float kf = g;
-    if (q < 208.f) {
-      z = k.c2 * __builtin_amdgcn_exp2f(q * k.c1);
-      g = pinv2 * (pinv - z);
-      h = __builtin_fmaf(z, -1.f / 3.f, g) * pinv2;
-      if (q < (float)(GTAB_RHO_MAX * GTAB_RHO_MAX)) {
-        float t = q * pinv * (float)GTAB_PER_UNIT;  // rho * 16
-        int i = (int)t;
-        i = i < (GTAB_N - 1) ? i : (GTAB_N - 1);
-        i = i > 0 ? i : 0;
-        float u = t - (float)i;
-        float4 cg = tab[2 * i];
-        float4 ch = tab[2 * i + 1];
-        g = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(cg.w, u, cg.z), u, cg.y), u, cg.x);
-        h = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(ch.w, u, ch.z), u, ch.y), u, ch.x);
-        bool zr = (r2 == 0.f);
-        g = zr ? 0.f : g;
-        z = zr ? 0.f : z;
-      }
-      near(g, z);
+                                                     float& eta, float& Kfar, Near&& near_cb) {
+    const Far f = far(r2, k);
+    float g = f.g, h = f.h, z = 0.f, kf = f.g;
+    if (f.q < k.q_far) {
+      near(f, r2, k, tab, g, h, z);
+      near_cb(g, z);
       kf = 0.f;
     }
     Kv = g;

@@ -229,6 +229,7 @@ __device__ __forceinline__ void p3d_block(const P3DKParams& p, const RegConsts& 
     for (int a = 0; a < ND; ++a) dacc[k][a] = 0.0;
   }
 
+  const RegConsts rv = loop_consts(rc);
   for (int ts = s0; ts < s1; ts += WG) {
     const int cnt = min(WG, s1 - ts);
     __syncthreads();
@@ -249,14 +250,17 @@ __device__ __forceinline__ void p3d_block(const P3DKParams& p, const RegConsts& 
 #pragma unroll
         for (int a = 0; a < NF; ++a) acc[k][a] = 0.f;
     }
+    // the LDS index runs in a VGPR too: an SGPR index would cost an
+    // SGPR-reading move per source to form the ds_read address
+    int jv = in_vgpr(0);
 #pragma unroll 2
-    for (int j = 0; j < cnt; ++j) {
-      const float4 sp = sm.pv[j];
-      const float4 sa = sm.a[j];
+    for (int j = 0; j < cnt; ++j, ++jv) {
+      const float4 sp = sm.pv[jv];
+      const float4 sa = sm.a[jv];
 #pragma unroll
       for (int k = 0; k < T; ++k)
-        p3d_pair<REG, MASK, ACC>(sp, sa, &sm.xy[DBL_ ? j : 0], &sm.zv[DBL_ ? j : 0], xt[k], at[k],
-                                 rc, sm.tab, acc[k], dacc[k]);
+        p3d_pair<REG, MASK, ACC>(sp, sa, &sm.xy[DBL_ ? jv : 0], &sm.zv[DBL_ ? jv : 0], xt[k], at[k],
+                                 rv, sm.tab, acc[k], dacc[k]);
     }
     if constexpr (ACC == ACC_COMP) {
 #pragma unroll
