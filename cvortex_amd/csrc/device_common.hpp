@@ -102,17 +102,22 @@ __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float x) {
 }
 
 // ---- regularisation kernels (reduced form) -------------------------------
-// eval(r2, K, lds_table, Kv, Kh, eta)
+// eval<SELF>(r2, K, lds_table, Kv, Kh, eta)
 //   Kv: velocity / stretching kernel, zero when r2 == 0 (self and
 //       coincident pairs are excluded, reference src/P3D.cpp:58,94,127)
 //   Kh: gradient kernel; Kh * 0 must be 0 when r2 == 0
 //   eta: PSE kernel, zero when r2 == 0
+// SELF = false (velocity only) drops the r2 == 0 test where Kv is finite
+// there: the velocity term Kv (d x a_s) is then exactly 0 for d = 0, and the
+// test is a compare and a select per pair, which are not dual-issued.
 template <int REG>
 struct RegKernel;
 
 template <>
 struct RegKernel<REG_SINGULAR> {
-  // Kv = r^-3, Kh = r^-5 (x3 in the epilogue)
+  // Kv = r^-3, Kh = r^-5 (x3 in the epilogue); infinite at r2 = 0, so the
+  // test stays for every SELF
+  template <bool SELF = true>
   static __device__ __forceinline__ void eval(float r2, const RegConsts&, const float4*,
                                               float& Kv, float& Kh, float& eta) {
     float y = __builtin_amdgcn_rsqf(r2);
@@ -133,6 +138,7 @@ struct RegKernel<REG_WINCKELMANS> {
   //   Kv  = G(rho)     = (q + 2.5) y^5
   //   Kh  = H(rho) / 3 = (q + 3.5) y^7
   //   eta = y^9
+  template <bool SELF = true>
   static __device__ __forceinline__ void eval(float r2, const RegConsts& k, const float4*,
                                               float& Kv, float& Kh, float& eta) {
     float s = __builtin_fmaf(r2, k.inv_s2, 1.f);
@@ -143,9 +149,14 @@ struct RegKernel<REG_WINCKELMANS> {
     float kv = ((s + 1.5f) * y2) * y3;
     Kh = ((s + 2.5f) * y2) * y5;
     float e = (y5 * y2) * y2;
-    bool z = (r2 == 0.f);
-    Kv = z ? 0.f : kv;
-    eta = z ? 0.f : e;
+    if constexpr (SELF) {
+      bool z = (r2 == 0.f);
+      Kv = z ? 0.f : kv;
+      eta = z ? 0.f : e;
+    } else {
+      Kv = kv;
+      eta = e;
+    }
   }
 };
 
@@ -154,6 +165,7 @@ struct RegKernel<REG_PLANETARY> {
   // Reduced units, q = rho^2.  Inside the core (q < 1): g = rho^3, zeta = 3,
   // so G = 1 and H = 0; outside: g = 1, zeta = 0, so G = q^-3/2 and
   // H / 3 = q^-5/2.  The epilogue applies |sigma|^-3 and 3 |sigma|^-5.
+  template <bool SELF = true>
   static __device__ __forceinline__ void eval(float r2, const RegConsts& k, const float4*,
                                               float& Kv, float& Kh, float& eta) {
     float q = r2 * k.inv_s2;
@@ -162,7 +174,7 @@ struct RegKernel<REG_PLANETARY> {
     float kv = y * y2;
     float kh = kv * y2;
     bool in = q < 1.f;
-    bool z = (r2 == 0.f);
+    bool z = SELF && (r2 == 0.f);
     Kv = z ? 0.f : (in ? 1.f : kv);
     Kh = in ? 0.f : kh;
     eta = 0.f;
@@ -245,6 +257,7 @@ struct RegKernel<REG_GAUSSIAN> {
     Kfar = kf;
   }
 
+  template <bool SELF = true>
   static __device__ __forceinline__ void eval(float r2, const RegConsts& k, const float4* tab,
                                               float& Kv, float& Kh, float& eta) {
     float kf;

@@ -94,7 +94,7 @@ __device__ __forceinline__ void p3d_pair(const float4 sp, const float4 sa, const
       }
     });
   } else {
-    RegKernel<REG>::eval(r2, rc, tab, Kv, Kh, eta);
+    RegKernel<REG>::template eval<(MASK & (OUT_DVORT | OUT_VISC)) != 0>(r2, rc, tab, Kv, Kh, eta);
   }
   if constexpr ((MASK & (OUT_VEL | OUT_DVORT)) != 0) {
     // c = d x a_s
