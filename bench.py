@@ -174,6 +174,25 @@ def bench_ring(args, cv, D, W, o, stream):
         variants["vel_dvort_visc"] = {"interactions_per_s": pairs / (s3 * 1e-3), "kernel_ms": k3,
                                       "tflops": roofline(pairs, F3, k3)["achieved"],
                                       "frac_of_peak": roofline(pairs, F3, k3)["frac"]}
+    if "c_api" in args.variants:
+        # the host-array boundary: gather from the caller's pointer arrays,
+        # copies over PCIe both ways, kernels, all inside the timed call
+        cv.P3D_M2M_fused(p, p, "gaussian", sigma, 0.0, vel=True, dvort=True)
+        t0 = time.perf_counter()
+        cv.P3D_M2M_fused(p, p, "gaussian", sigma, 0.0, vel=True, dvort=True)
+        fused_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        cv.P3D_M2M_vel(p, p["coord"], "gaussian", sigma)
+        vel_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        cv.P3D_M2M_dvort(p, p, "gaussian", sigma)
+        dvort_s = time.perf_counter() - t0
+        variants["c_api_host_arrays"] = {
+            "fused_ms": fused_s * 1e3, "vel_ms": vel_s * 1e3, "dvort_ms": dvort_s * 1e3,
+            "fused_interactions_per_s": pairs / fused_s,
+            "vel_then_dvort_interactions_per_s": pairs / (vel_s + dvort_s)}
+        log(f"  C API on host arrays: fused {fused_s * 1e3:.1f} ms, vel {vel_s * 1e3:.1f} ms, "
+            f"dvort {dvort_s * 1e3:.1f} ms")
     err = None
     if n_err(args, 4096) > 0:
         rng = np.random.default_rng(0x5EED)
@@ -664,7 +683,7 @@ def main():
     ap.add_argument("--cpu-samples", type=int, default=512)
     ap.add_argument("--rk4-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variants", default="plain,visc")
+    ap.add_argument("--variants", default="plain,visc,c_api")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -240,13 +240,22 @@ struct RegKernel<REG_GAUSSIAN> {
     }
   }
 
-  template <class Near>
+  // PIN: keep the near range a branch.  With no work of its own in
+  // near_cb the compiler if-converts it, and every pair then pays the exp (a
+  // transcendental, about three issue slots) and the selects.  Where near_cb
+  // does FP64 sums it stays a branch by itself, and the pin would only stop
+  // the loop around it from unrolling.  Measured on the 2^20 ring: the
+  // compensated vel-only kernel 593 -> 515 ms with the pin; the plain FP32
+  // vel+dvort kernel (two targets per lane) 680 -> 700 ms, so eval() does
+  // not pin.
+  template <bool PIN = false, class Near>
   static __device__ __forceinline__ void eval_ranges(float r2, const RegConsts& k,
                                                      const float4* tab, float& Kv, float& Kh,
                                                      float& eta, float& Kfar, Near&& near_cb) {
     const Far f = far(r2, k);
     float g = f.g, h = f.h, z = 0.f, kf = f.g;
     if (f.q < k.q_far) {
+      if constexpr (PIN) asm volatile("");
       near(f, r2, k, tab, g, h, z);
       near_cb(g, z);
       kf = 0.f;

@@ -75,7 +75,9 @@ __device__ __forceinline__ void p3d_pair(const float4 sp, const float4 sa, const
   const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
   float Kv, Kh, eta, Kfar = 0.f;
   if constexpr (R::SPLIT) {
-    RegKernel<REG_GAUSSIAN>::eval_ranges(r2, rc, tab, Kv, Kh, eta, Kfar, [&](float kn, float en) {
+    constexpr bool PIN = (MASK & (OUT_DVORT | OUT_VISC)) == 0;  // no FP64 near sums
+    RegKernel<REG_GAUSSIAN>::template eval_ranges<PIN>(r2, rc, tab, Kv, Kh, eta, Kfar,
+                                                       [&](float kn, float en) {
       // FP64 source strengths are read from LDS here, on the rare path (a
       // conversion of sa would be speculated out of the branch)
       const double2 sxy = *pxy, szv = *pzv;
