@@ -49,7 +49,7 @@ FLOPS_PER_PAIR = {
     ("p3d", "vel+dvort+visc", "winckelmans"): 77, ("p3d", "vel+dvort+visc", "gaussian"): 94,
     ("f3d", "vel", "singular"): 56, ("f3d", "dvort", "singular"): 76,
     ("f3d", "vel+dvort", "singular"): 100,
-    ("p2d", "vel", "planetary"): 14,
+    ("p2d", "vel", "planetary"): 14, ("p2d", "vel+visc", "gaussian"): 21,
     # vorticity field, counted with the same convention: d 3, box test 5
     # (3 compares, 2 ands), r^2 5, rho^2 1, zeta (exp 2), select 1, 3 FMA 6
     ("vort", "vort", "gaussian"): 23,
@@ -443,6 +443,15 @@ def bench_sheet(args, cv, D, W, o, stream):
         s2, k2 = tm.run(lambda: D.p2d_eval(prob2, stream), max(1, args.steps - 1), 1)
         variants["plain_fp32"] = {"interactions_per_s": pairs / (s2 * 1e-3), "kernel_ms": k2}
         plainV = v2.download()
+    if "gaussian" in args.variants:
+        # supplementary: the same sheet with the Gaussian kernel, vel + visc fused
+        vg, eg = D.DeviceArray((n, 2)), D.DeviceArray((n,))
+        prob3 = D.P2DProblem(ds.addr, n, ds.addr, n, vg.addr, eg.addr, 3, sigma, nu, cv.ACC_COMP)
+        s3, k3 = tm.run(lambda: D.p2d_eval(prob3, stream), max(1, args.steps - 1), 1)
+        F3 = FLOPS_PER_PAIR[("p2d", "vel+visc", "gaussian")]
+        variants["gaussian_vel_visc"] = {"interactions_per_s": pairs / (s3 * 1e-3), "kernel_ms": k3,
+                                         "frac_of_peak": roofline(pairs, F3, k3)["frac"]}
+        log(f"  Gaussian vel+visc: {k3:.1f} ms")
     rng = np.random.default_rng(0x5EED + 5)
     idx = np.sort(rng.choice(n, size=min(n_err(args, 8192) or 8192, n), replace=False)).astype(np.int32)
     v64, _ = o.f64_P2D(p, p, "planetary", sigma, idx=idx)

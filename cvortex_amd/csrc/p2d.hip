@@ -63,28 +63,43 @@ struct Reg2D<REG_PLANETARY> {
   }
 };
 
-// Gaussian: g = 1 - exp(-q/2) -> K = (1 - exp(-q/2)) / q (x 1/sigma^2); for
-// q < 1 the series 1/2 sum_m (-q/2)^m / (m+1)! avoids the cancellation.
-// eta_2D = exp(-q/2).
+// Gaussian: g = 1 - exp(-q/2) -> K = (1 - exp(-q/2)) / q (x 1/sigma^2),
+// eta_2D = exp(-q/2).  Three nested ranges of q, so that a wave runs only
+// the ranges its lanes need (a wave of nearby targets is mostly in the first):
+//   q >= 208: exp(-q/2) < 2^-150 rounds to 0 in FP32, so K = 1/q and eta = 0
+//             (what the general form gives there);
+//   q >= 1:   K = (1 - e)/q, eta = e;
+//   q < 1:    the series 1/2 sum_{m<9} (-q/2)^m / (m+1)! avoids the
+//             cancellation of 1 - e; eta is zeroed at r2 = 0.
+// The empty asm keeps the first test a branch: if-converted, every pair
+// would pay the exp and the series.
 template <>
 struct Reg2D<REG_GAUSSIAN> {
   static __device__ __forceinline__ void eval(float r2, const RegConsts& k, float& K, float& eta) {
-    float q = r2 * k.inv_s2;
-    float e = __builtin_amdgcn_exp2f(q * k.c1);
-    float far = (1.f - e) * __builtin_amdgcn_rcpf(q);
-    float x = -0.5f * q;
-    // 1/2 * sum_{m=0}^{8} x^m / (m+1)!
-    float s = 1.f / 362880.f;
-    s = __builtin_fmaf(s, x, 1.f / 40320.f);
-    s = __builtin_fmaf(s, x, 1.f / 5040.f);
-    s = __builtin_fmaf(s, x, 1.f / 720.f);
-    s = __builtin_fmaf(s, x, 1.f / 120.f);
-    s = __builtin_fmaf(s, x, 1.f / 24.f);
-    s = __builtin_fmaf(s, x, 1.f / 6.f);
-    s = __builtin_fmaf(s, x, 0.5f);
-    s = __builtin_fmaf(s, x, 1.f);
-    K = (q < 1.f) ? 0.5f * s : far;
-    eta = (r2 == 0.f) ? 0.f : e;
+    const float q = r2 * k.inv_s2;
+    const float y = __builtin_amdgcn_rcpf(q);
+    K = y;
+    eta = 0.f;
+    if (q < k.q_far) {
+      asm volatile("");
+      const float e = __builtin_amdgcn_exp2f(q * k.c1);
+      K = (1.f - e) * y;
+      eta = e;
+      if (q < 1.f) {
+        const float x = -0.5f * q;
+        float s = 1.f / 362880.f;
+        s = __builtin_fmaf(s, x, 1.f / 40320.f);
+        s = __builtin_fmaf(s, x, 1.f / 5040.f);
+        s = __builtin_fmaf(s, x, 1.f / 720.f);
+        s = __builtin_fmaf(s, x, 1.f / 120.f);
+        s = __builtin_fmaf(s, x, 1.f / 24.f);
+        s = __builtin_fmaf(s, x, 1.f / 6.f);
+        s = __builtin_fmaf(s, x, 0.5f);
+        s = __builtin_fmaf(s, x, 1.f);
+        K = 0.5f * s;
+        eta = (r2 == 0.f) ? 0.f : e;
+      }
+    }
   }
 };
 
@@ -134,6 +149,7 @@ __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
 
   const int s0 = blockIdx.y * p.split_len;
   const int s1 = min(s0 + p.split_len, p.n_src);
+  const RegConsts rv = loop_consts(p.rc);
   for (int ts = s0; ts < s1; ts += WG) {
     const int cnt = min(WG, s1 - ts);
     __syncthreads();
@@ -145,16 +161,17 @@ __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
 #pragma unroll
         for (int a = 0; a < NA; ++a) acc[k][a] = 0.f;
     }
+    int jv = in_vgpr(0);  // LDS index in a VGPR (no SGPR operand per source)
 #pragma unroll 4
-    for (int j = 0; j < cnt; ++j) {
-      const float4 s = s_src[j];
+    for (int j = 0; j < cnt; ++j, ++jv) {
+      const float4 s = s_src[jv];
 #pragma unroll
       for (int k = 0; k < T; ++k) {
         const float dx = xt[k].x - s.x;
         const float dy = xt[k].y - s.y;
         const float r2 = __builtin_fmaf(dx, dx, dy * dy);
         float K, eta;
-        Reg2D<REG>::eval(r2, p.rc, K, eta);
+        Reg2D<REG>::eval(r2, rv, K, eta);
         if constexpr ((MASK & OUT_VEL) != 0) {
           const float ka = K * s.z;
           acc[k][0] = __builtin_fmaf(ka, dy, acc[k][0]);
