@@ -443,6 +443,16 @@ def bench_sheet(args, cv, D, W, o, stream):
         s2, k2 = tm.run(lambda: D.p2d_eval(prob2, stream), max(1, args.steps - 1), 1)
         variants["plain_fp32"] = {"interactions_per_s": pairs / (s2 * 1e-3), "kernel_ms": k2}
         plainV = v2.download()
+    fp64V = None
+    if "fp64final" in args.variants:
+        # compensated tiles in 8 source partitions, the partitions added in
+        # FP64 by the reduction kernel (two targets per lane)
+        v4 = D.DeviceArray((n, 2))
+        prob4 = D.P2DProblem(ds.addr, n, ds.addr, n, v4.addr, None, 2, sigma, nu, cv.ACC_COMP)
+        s4, k4 = tm.run(lambda: D.p2d_eval(prob4, stream, shape=(2, 8)), max(1, args.steps - 1), 1)
+        variants["compensated_fp64_partitions"] = {"interactions_per_s": pairs / (s4 * 1e-3),
+                                                   "kernel_ms": k4, "shape": "T=2, 8 partitions"}
+        fp64V = v4.download()
     if "gaussian" in args.variants:
         # supplementary: the same sheet with the Gaussian kernel, vel + visc fused
         vg, eg = D.DeviceArray((n, 2)), D.DeviceArray((n,))
@@ -458,6 +468,8 @@ def bench_sheet(args, cv, D, W, o, stream):
     err = {"samples": int(len(idx)), "vel": err_stats(V[idx], v64)}
     if "plain" in args.variants:
         err["plain_fp32"] = {"vel": err_stats(plainV[idx], v64)}
+    if fp64V is not None:
+        err["compensated_fp64_partitions"] = {"vel": err_stats(fp64V[idx], v64)}
     # visc: planetary has no eta -> exactly zero (reference VortFunc.cpp:52-62)
     vz = cv.P2D_M2M_visc_dvort(p[:4096], p[:4096], "planetary", sigma, nu)
     m = min(args.cpu_samples, n)
@@ -692,7 +704,7 @@ def main():
     ap.add_argument("--cpu-samples", type=int, default=512)
     ap.add_argument("--rk4-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variants", default="plain,visc,c_api")
+    ap.add_argument("--variants", default="plain,visc,c_api,fp64final,gaussian")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
