@@ -112,11 +112,12 @@ __global__ __launch_bounds__(WG) void f3d_kernel(F3DKParams p) {
 #pragma unroll
         for (int a = 0; a < NA; ++a) acc[k][a] = 0.f;
     }
+    int jv = in_vgpr(0);  // LDS index in a VGPR (no SGPR operand per filament)
 #pragma unroll 2
-    for (int j = 0; j < cnt; ++j) {
-      const float4 fa = s_a[j];
-      const float4 fb = s_b[j];
-      const float4 f0 = s_r0[j];
+    for (int j = 0; j < cnt; ++j, ++jv) {
+      const float4 fa = s_a[jv];
+      const float4 fb = s_b[jv];
+      const float4 f0 = s_r0[jv];
 #pragma unroll
       for (int k = 0; k < T; ++k) {
         const float r1x = xt[k].x - fa.x, r1y = xt[k].y - fa.y, r1z = xt[k].z - fa.z;
@@ -141,10 +142,12 @@ __global__ __launch_bounds__(WG) void f3d_kernel(F3DKParams p) {
         float kk = fa.w * num * __builtin_amdgcn_rcpf(den);
         if constexpr ((MASK & OUT_DVORT) != 0) {
           const float rcc = __builtin_amdgcn_rsqf(cc);
-          const float cn = cc * rcc;  // |c|
+          const float cn = cc * rcc;  // |c|; 0 * inf = NaN when c = 0
           const float p12 = __builtin_fmaf(f0.x, r1x + r2x, __builtin_fmaf(f0.y, r1y + r2y, f0.z * (r1z + r2z)));
           float B = -fb.w * cn * p12 * __builtin_amdgcn_rcpf(P * sum);
-          const bool ok = (cc > 0.f) && (__builtin_fabsf(kk) < __builtin_inff()) &&
+          // B is NaN whenever c = 0 (target on the filament's line), so the
+          // finiteness of B also drops those pairs
+          const bool ok = (__builtin_fabsf(kk) < __builtin_inff()) &&
                           (__builtin_fabsf(B) < __builtin_inff());
           B = ok ? B : 0.f;
           const float kd = ok ? kk : 0.f;

@@ -164,6 +164,9 @@ __global__ __launch_bounds__(WG) void vort_kernel(VortKParams p) {
     for (int a = 0; a < 3; ++a) { acc[k][a] = 0.f; sh[k][a] = 0.f; sl[k][a] = 0.f; }
 
   const float c = p.cutoff;
+  // per-pair uniform values in VGPRs (an SGPR operand blocks dual issue)
+  const float cv = in_vgpr(c);
+  const RegConsts rv = loop_consts(p.rc);
   const int s0 = blockIdx.y * p.split_len;
   const int s1 = min(s0 + p.split_len, p.n_src);
   for (int ts = s0; ts < s1; ts += WG) {
@@ -199,18 +202,19 @@ __global__ __launch_bounds__(WG) void vort_kernel(VortKParams p) {
         if (apart) continue;
       }
       const int j1 = min(cnt, j0 + 64);
+      int jv = in_vgpr(j0);  // LDS index in a VGPR
 #pragma unroll 2
-      for (int j = j0; j < j1; ++j) {
-        const float4 sp = s_pv[j];
-        const float4 sa = s_a[j];
+      for (int j = j0; j < j1; ++j, ++jv) {
+        const float4 sp = s_pv[jv];
+        const float4 sa = s_a[jv];
 #pragma unroll
         for (int k = 0; k < T; ++k) {
           const float dx = sp.x - xt[k].x;
           const float dy = sp.y - xt[k].y;
           const float dz = sp.z - xt[k].z;
-          const bool in = (__builtin_fabsf(dx) < c) && (__builtin_fabsf(dy) < c) &&
-                          (__builtin_fabsf(dz) < c);
-          float z = Zeta<REG>::eval(dx, dy, dz, p.rsigma, p.sigma, p.divide != 0, p.rc);
+          const bool in = (__builtin_fabsf(dx) < cv) && (__builtin_fabsf(dy) < cv) &&
+                          (__builtin_fabsf(dz) < cv);
+          float z = Zeta<REG>::eval(dx, dy, dz, p.rsigma, p.sigma, p.divide != 0, rv);
           z = in ? z : 0.f;
           acc[k][0] = __builtin_fmaf(z, sa.x, acc[k][0]);
           acc[k][1] = __builtin_fmaf(z, sa.y, acc[k][1]);
