@@ -77,8 +77,8 @@ __global__ void rk4_stage(int n, const float4* __restrict__ y0p, const float4* _
                           int first) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float vx = kv[3 * i], vy = kv[3 * i + 1], vz = kv[3 * i + 2];
-  const float dx = kd[3 * i], dy = kd[3 * i + 1], dz = kd[3 * i + 2];
+  const float vx = kv[3 * (size_t)i], vy = kv[3 * (size_t)i + 1], vz = kv[3 * (size_t)i + 2];
+  const float dx = kd[3 * (size_t)i], dy = kd[3 * (size_t)i + 1], dz = kd[3 * (size_t)i + 2];
   const float4 p = y0p[i], a = y0a[i];
   ysp[i] = make_float4(__builtin_fmaf(c, vx, p.x), __builtin_fmaf(c, vy, p.y),
                        __builtin_fmaf(c, vz, p.z), p.w);
@@ -104,12 +104,12 @@ __global__ void rk4_final(int n, float4* __restrict__ y0p, float4* __restrict__ 
   if (i >= n) return;
   const float4 sp = Sp[i], sa = Sa[i];
   float4 p = y0p[i], a = y0a[i];
-  p.x = __builtin_fmaf(h, sp.x + kv[3 * i], p.x);
-  p.y = __builtin_fmaf(h, sp.y + kv[3 * i + 1], p.y);
-  p.z = __builtin_fmaf(h, sp.z + kv[3 * i + 2], p.z);
-  a.x = __builtin_fmaf(h, sa.x + kd[3 * i], a.x);
-  a.y = __builtin_fmaf(h, sa.y + kd[3 * i + 1], a.y);
-  a.z = __builtin_fmaf(h, sa.z + kd[3 * i + 2], a.z);
+  p.x = __builtin_fmaf(h, sp.x + kv[3 * (size_t)i], p.x);
+  p.y = __builtin_fmaf(h, sp.y + kv[3 * (size_t)i + 1], p.y);
+  p.z = __builtin_fmaf(h, sp.z + kv[3 * (size_t)i + 2], p.z);
+  a.x = __builtin_fmaf(h, sa.x + kd[3 * (size_t)i], a.x);
+  a.y = __builtin_fmaf(h, sa.y + kd[3 * (size_t)i + 1], a.y);
+  a.z = __builtin_fmaf(h, sa.z + kd[3 * (size_t)i + 2], a.z);
   y0p[i] = p;
   y0a[i] = a;
 }

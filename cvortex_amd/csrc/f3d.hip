@@ -47,16 +47,16 @@ template <int MASK>
 __device__ __forceinline__ void f3d_finish(const double* a, float3 at, const F3DKParams& p, int i) {
   constexpr int D = (MASK & OUT_VEL) ? 3 : 0;
   if constexpr ((MASK & OUT_VEL) != 0) {
-    p.out_vel[3 * i + 0] = (float)a[0];
-    p.out_vel[3 * i + 1] = (float)a[1];
-    p.out_vel[3 * i + 2] = (float)a[2];
+    p.out_vel[3 * (size_t)i + 0] = (float)a[0];
+    p.out_vel[3 * (size_t)i + 1] = (float)a[1];
+    p.out_vel[3 * (size_t)i + 2] = (float)a[2];
   }
   if constexpr ((MASK & OUT_DVORT) != 0) {
     const double B = a[D], Sx = a[D + 1], Sy = a[D + 2], Sz = a[D + 3];
     const double ax = at.x, ay = at.y, az = at.z;
-    p.out_dvort[3 * i + 0] = (float)(ax * B - (Sy * az - Sz * ay));
-    p.out_dvort[3 * i + 1] = (float)(ay * B - (Sz * ax - Sx * az));
-    p.out_dvort[3 * i + 2] = (float)(az * B - (Sx * ay - Sy * ax));
+    p.out_dvort[3 * (size_t)i + 0] = (float)(ax * B - (Sy * az - Sz * ay));
+    p.out_dvort[3 * (size_t)i + 1] = (float)(ay * B - (Sz * ax - Sx * az));
+    p.out_dvort[3 * (size_t)i + 2] = (float)(az * B - (Sx * ay - Sy * ax));
   }
 }
 

@@ -119,8 +119,8 @@ __device__ __forceinline__ void p2d_finish(const double* a, float4 t, const P2DK
   constexpr int V = 0;
   constexpr int E = (MASK & OUT_VEL) ? 2 : 0;
   if constexpr ((MASK & OUT_VEL) != 0) {
-    p.out_vel[2 * i + 0] = (float)(p.sc_vel * a[V + 0]);
-    p.out_vel[2 * i + 1] = (float)(-p.sc_vel * a[V + 1]);
+    p.out_vel[2 * (size_t)i + 0] = (float)(p.sc_vel * a[V + 0]);
+    p.out_vel[2 * (size_t)i + 1] = (float)(-p.sc_vel * a[V + 1]);
   }
   if constexpr ((MASK & OUT_VISC) != 0) {
     p.out_visc[i] = (float)(p.sc_visc * ((double)t.w * a[E + 0] - (double)t.z * a[E + 1]));

@@ -154,9 +154,9 @@ __device__ __forceinline__ void p3d_finish(const double* a, float3 at, float vt,
                                            int i) {
   using L = P3DLayout<MASK>;
   if constexpr ((MASK & OUT_VEL) != 0) {
-    out_vel[3 * i + 0] = (float)(sc.vel * a[L::VEL + 0]);
-    out_vel[3 * i + 1] = (float)(sc.vel * a[L::VEL + 1]);
-    out_vel[3 * i + 2] = (float)(sc.vel * a[L::VEL + 2]);
+    out_vel[3 * (size_t)i + 0] = (float)(sc.vel * a[L::VEL + 0]);
+    out_vel[3 * (size_t)i + 1] = (float)(sc.vel * a[L::VEL + 1]);
+    out_vel[3 * (size_t)i + 2] = (float)(sc.vel * a[L::VEL + 2]);
   }
   if constexpr ((MASK & OUT_DVORT) != 0) {
     const double ax = at.x, ay = at.y, az = at.z;
@@ -164,15 +164,15 @@ __device__ __forceinline__ void p3d_finish(const double* a, float3 at, float vt,
     const double wx = ay * Az - az * Ay;
     const double wy = az * Ax - ax * Az;
     const double wz = ax * Ay - ay * Ax;
-    out_dvort[3 * i + 0] = (float)(sc.dG * wx + sc.dH * a[L::H + 0]);
-    out_dvort[3 * i + 1] = (float)(sc.dG * wy + sc.dH * a[L::H + 1]);
-    out_dvort[3 * i + 2] = (float)(sc.dG * wz + sc.dH * a[L::H + 2]);
+    out_dvort[3 * (size_t)i + 0] = (float)(sc.dG * wx + sc.dH * a[L::H + 0]);
+    out_dvort[3 * (size_t)i + 1] = (float)(sc.dG * wy + sc.dH * a[L::H + 1]);
+    out_dvort[3 * (size_t)i + 2] = (float)(sc.dG * wz + sc.dH * a[L::H + 2]);
   }
   if constexpr ((MASK & OUT_VISC) != 0) {
     const double v = vt, e = a[L::EV];
-    out_visc[3 * i + 0] = (float)(sc.visc * (v * a[L::E + 0] - (double)at.x * e));
-    out_visc[3 * i + 1] = (float)(sc.visc * (v * a[L::E + 1] - (double)at.y * e));
-    out_visc[3 * i + 2] = (float)(sc.visc * (v * a[L::E + 2] - (double)at.z * e));
+    out_visc[3 * (size_t)i + 0] = (float)(sc.visc * (v * a[L::E + 0] - (double)at.x * e));
+    out_visc[3 * (size_t)i + 1] = (float)(sc.visc * (v * a[L::E + 1] - (double)at.y * e));
+    out_visc[3 * (size_t)i + 2] = (float)(sc.visc * (v * a[L::E + 2] - (double)at.z * e));
   }
 }
 

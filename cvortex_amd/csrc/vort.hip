@@ -241,7 +241,7 @@ __global__ __launch_bounds__(WG) void vort_kernel(VortKParams p) {
     if (p.n_split == 1) {
       const int o = p.tgt_perm ? p.tgt_perm[i] : i;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) p.out[3 * o + a] = (float)(p.scale * d[a]);
+      for (int a = 0; a < 3; ++a) p.out[3 * (size_t)o + a] = (float)(p.scale * d[a]);
     } else {
 #pragma unroll
       for (int a = 0; a < 3; ++a) p.partial[((size_t)blockIdx.y * 3 + a) * p.n_tgt + i] = d[a];
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(256) void vort_reduce(VortKParams p) {
     for (int a = 0; a < 3; ++a) d[a] += p.partial[((size_t)s * 3 + a) * p.n_tgt + i];
   const int o = p.tgt_perm ? p.tgt_perm[i] : i;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) p.out[3 * o + a] = (float)(p.scale * d[a]);
+  for (int a = 0; a < 3; ++a) p.out[3 * (size_t)o + a] = (float)(p.scale * d[a]);
 }
 
 // Sorted copies of the sources: pv_out[i] = pv[perm[i]], a_out[i] = a[perm[i]].
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(256) void relax_kernel(float4* __restrict__ a,
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const float4 ov = a[i];
-  const float wx = omega[3 * i + 0], wy = omega[3 * i + 1], wz = omega[3 * i + 2];
+  const float wx = omega[3 * (size_t)i + 0], wy = omega[3 * (size_t)i + 1], wz = omega[3 * (size_t)i + 2];
   const float aw =
       __fsqrt_rn(rn_add(rn_add(rn_mul(wx, wx), rn_mul(wy, wy)), rn_mul(wz, wz)));
   const float aa = __fsqrt_rn(

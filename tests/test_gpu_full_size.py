@@ -1,0 +1,131 @@
+"""The five benchmark workloads at their full sizes on the GPU.
+
+Each test evaluates the whole problem once (through the host C API where
+the reference has one: gather, copies, kernels) and checks it in two ways.
+The checks do not depend on the problem size:
+  * sampled targets against the FP64 oracle: relative L2 <= 1e-5, and for
+    the ring also RMS per-target relative error <= 1e-6;
+  * exact invariants of the arithmetic:
+      - strengths scaled by 2 give outputs scaled by exactly 2 (every
+        product and sum scales exactly);
+      - a negative regularisation radius negates dvort exactly (the
+        reference's signed sigma^3, src/P3D.cpp:102);
+      - the persistent multi-cloud launch equals one launch per cloud;
+      - planetary PSE is exactly zero (reference src/VortFunc.cpp:52-62).
+Workload generators: cvortex_amd/workloads.py."""
+import numpy as np
+import pytest
+
+import cvortex_amd as cv
+from cvortex_amd import device as D
+from cvortex_amd import workloads as W
+from gpuutil import assert_parity, case, have_gpu
+import oracle
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not have_gpu():
+        pytest.fail("no HIP device: GPU tests need one")
+    cv.set_accumulation(cv.ACC_COMP)
+
+
+def sample(n, m, seed):
+    return np.sort(np.random.default_rng(seed).choice(n, size=m, replace=False)).astype(np.int32)
+
+
+def rms_rel(got, ref):
+    r = oracle.per_target_rel(got, ref, floor=1e-30)
+    return float(np.sqrt(np.mean(r * r)))
+
+
+def test_ring_2pow20(orc):
+    p, sigma, _ = W.ring()
+    n = len(p)
+    case(f"ring n={n}: fused vel+dvort, Gaussian")
+    v, d, _ = cv.P3D_M2M_fused(p, p, "gaussian", sigma, 0.0, vel=True, dvort=True)
+    idx = sample(n, 256, 1)
+    v64, d64, _ = orc.f64_P3D(p, p, "gaussian", sigma, idx=idx, gauss_mode=1)
+    for name, g, r in (("vel", v[idx], v64), ("dvort", d[idx], d64)):
+        assert np.all(np.isfinite(g))
+        e, rms = oracle.rel_l2(g, r), rms_rel(g, r)
+        print(f"  {name}: relL2 vs FP64 {e:.2e}, RMS per target {rms:.2e}", flush=True)
+        assert e <= TOL
+        assert rms <= 1e-6, f"{name}: RMS relative error {rms:.2e} > 1e-6"
+    case("ring: negative sigma negates dvort exactly, vel unchanged")
+    v2, d2, _ = cv.P3D_M2M_fused(p, p, "gaussian", -sigma, 0.0, vel=True, dvort=True)
+    assert np.array_equal(v2, v)
+    assert np.array_equal(d2, -d)
+
+
+def test_cube_8192(orc):
+    p, sigma = W.cube()
+    case(f"cube n={len(p)}: vel, Winckelmans, all targets")
+    v = cv.P3D_M2M_vel(p, p["coord"], "winckelmans", sigma)
+    assert_parity("cube vel vs reference restatement", v,
+                  orc.P3D_M2M_vel(p, p["coord"], "winckelmans", sigma), TOL)
+    v64 = orc.f64_P3D_vel_pts(p, p["coord"], "winckelmans", sigma)
+    assert oracle.rel_l2(v, v64) <= TOL
+    case("cube: doubled strengths double vel exactly")
+    q = p.copy()
+    q["vorticity"] *= 2.0
+    assert np.array_equal(cv.P3D_M2M_vel(q, p["coord"], "winckelmans", sigma), 2.0 * v)
+
+
+def test_lattice_16384_on_2pow20(orc):
+    f, t = W.lattice()
+    case(f"lattice {len(f)} filaments x {len(t)} targets: F3D vel+dvort")
+    v, d = cv.F3D_M2M_fused(f, t)
+    idx = sample(len(t), 256, 3)
+    v64, d64 = orc.f64_F3D(f, t, idx=idx)
+    for name, g, r in (("vel", v[idx], v64), ("dvort", d[idx], d64)):
+        assert np.all(np.isfinite(g))
+        e = oracle.rel_l2(g, r)
+        print(f"  {name}: relL2 vs FP64 {e:.2e}", flush=True)
+        assert e <= TOL
+    case("lattice: doubled circulations double vel and dvort exactly")
+    g2 = f.copy()
+    g2["strength"] *= 2.0
+    v2, d2 = cv.F3D_M2M_fused(g2, t)
+    assert np.array_equal(v2, 2.0 * v)
+    assert np.array_equal(d2, 2.0 * d)
+
+
+def test_clouds_512(orc):
+    p, off, sig = W.clouds()
+    n = len(p)
+    case(f"512 clouds, {n} particles: persistent vel+dvort, Winckelmans")
+    pv, a = D.p3d_soa(p)
+    dpv, da = D.DeviceArray.from_host(pv), D.DeviceArray.from_host(a)
+    b = D.Batch(off, sig, "winckelmans", outputs=cv.OUT_VEL | cv.OUT_DVORT)
+    vel, dv = D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
+    b.eval(dpv, da, vel, dv)
+    V, Dv = vel.download(), dv.download()
+    case("clouds: one plain launch per cloud")
+    vel2, dv2 = D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
+    b.eval_per_cloud(dpv, da, vel2, dv2, plain=True)
+    assert_parity("persistent vs per-cloud vel", V, vel2.download(), 1e-6)
+    assert_parity("persistent vs per-cloud dvort", Dv, dv2.download(), 1e-6)
+    for c in (0, 101, 257, 511):  # sampled targets of four clouds against FP64
+        lo, hi = int(off[c]), int(off[c + 1])
+        cl = np.ascontiguousarray(p[lo:hi])
+        idx = sample(hi - lo, 64, c)
+        v64, d64, _ = orc.f64_P3D(cl, cl, "winckelmans", float(sig[c]), idx=idx)
+        assert oracle.rel_l2(V[lo:hi][idx], v64) <= TOL
+        assert oracle.rel_l2(Dv[lo:hi][idx], d64) <= TOL
+
+
+def test_sheet_2pow22(orc):
+    p, sigma, nu = W.sheet()
+    n = len(p)
+    case(f"sheet n={n}: P2D vel + visc, planetary")
+    v, e = cv.P2D_M2M_fused(p, p, "planetary", sigma, nu)
+    assert np.all(e == 0.0)
+    idx = sample(n, 256, 5)
+    v64, _ = orc.f64_P2D(p, p, "planetary", sigma, idx=idx)
+    err = oracle.rel_l2(v[idx], v64)
+    print(f"  vel: relL2 vs FP64 {err:.2e}", flush=True)
+    assert np.all(np.isfinite(v)) and err <= TOL
