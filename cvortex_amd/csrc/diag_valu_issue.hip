@@ -19,13 +19,14 @@
 namespace {
 
 constexpr int ITERS = 1024;
-constexpr int NFORMS = 14;
+constexpr int NFORMS = 16;
 const char* const kNames[NFORMS] = {
     "v_sub_f32 v,v (reference)", "v_fma_f32 v,v,v,v",       "v_fma_f32 v,-v,v,v",
     "v_mul_f32 v,literal",       "v_fma_f32 v,v,s,v",       "v_mul_f32 s,v",
     "v_max_f32 v,v",             "v_cmp_lt_f32 + cndmask",  "v_cvt_i32_f32",
     "v_fma_f64 v,v,v,v",         "v_rcp_f32",               "v_rsq_f32",
-    "v_exp_f32",                 "v_sub + v_mul (2 ops)"};
+    "v_exp_f32",                 "v_sub + v_mul (2 ops)",   "v_sub_f32, one chain",
+    "v_sub_f32, two chains"};
 
 template <int F>
 __device__ __forceinline__ void op(float& x, float y, float z, float s) {
@@ -58,7 +59,16 @@ __global__ __launch_bounds__(256) void probe(float* out, float s) {
   for (int k = 0; k < 4; ++k) d[k] = x[k];
   const double dy = y, dz = z;
   for (int it = 0; it < ITERS; ++it) {
-    if constexpr (F == 9) {
+    if constexpr (F == 14) {  // one dependent chain per lane
+#pragma unroll
+      for (int k = 0; k < 8; ++k) op<0>(x[0], y, z, s);
+    } else if constexpr (F == 15) {  // two interleaved chains per lane
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        op<0>(x[0], y, z, s);
+        op<0>(x[1], y, z, s);
+      }
+    } else if constexpr (F == 9) {
 #pragma unroll
       for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -79,9 +89,9 @@ __global__ __launch_bounds__(256) void probe(float* out, float s) {
 using Fn = void (*)(float*, float);
 const Fn kKernels[NFORMS] = {probe<0>, probe<1>, probe<2>,  probe<3>,  probe<4>,
                              probe<5>, probe<6>, probe<7>,  probe<8>,  probe<9>,
-                             probe<10>, probe<11>, probe<12>, probe<13>};
+                             probe<10>, probe<11>, probe<12>, probe<13>, probe<14>, probe<15>};
 // instructions per chain step of each form
-const double kOps[NFORMS] = {1, 1, 1, 1, 1, 1, 1, 2, 1, 1, 1, 1, 1, 2};
+const double kOps[NFORMS] = {1, 1, 1, 1, 1, 1, 1, 2, 1, 1, 1, 1, 1, 2, 1, 1};
 
 bool ok(hipError_t e, const char* what) {
   if (e == hipSuccess) return true;
