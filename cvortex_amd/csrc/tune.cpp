@@ -75,9 +75,18 @@ LaunchCfg choose_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt,
   // reuse instead (861 ms against 678 ms), so it keeps T = 2.  Keep T = 1
   // only when it still gives several workgroups per CU without source
   // partitions.
+  const bool many_blocks = (long long)(n_tgt + 255) / 256 >= 4LL * (n_cu > 0 ? n_cu : 256);
   if (kind == KIND_P3D && acc == ACC_COMP && p3d_wants_order(reg, n_src, n_tgt) && c.T == 2 &&
-      c.n_split == 1 &&
-      (long long)(n_tgt + 255) / 256 >= 4LL * (n_cu > 0 ? n_cu : 256))
+      c.n_split == 1 && many_blocks)
+    c.T = 1;
+  // Filaments with one output (vel or dvort alone): one target per lane fits
+  // in 48-50 VGPRs, eight waves per SIMD against five or six with two, and
+  // the extra waves pair more instructions for dual issue than the second
+  // target's reuse saves (16384 filaments x 2^20 targets: vel 16.9 ms against
+  // 17.6, dvort 23.1 against 24.4; the fused kernel keeps two, 24.4 against
+  // 25.1).
+  if (kind == KIND_F3D && (mask == OUT_VEL || mask == OUT_DVORT) && c.T == 2 &&
+      c.n_split == 1 && many_blocks)
     c.T = 1;
   return c;
 }
