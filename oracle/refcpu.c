@@ -427,7 +427,7 @@ bsv_V3f orc_F3D_S2S_dvort(const cvtx_F3D* self, const cvtx_P3D* ind) {
 	bsv_V3f t211 = v3_divs(r0, -powf(v3_norm(v3_cross(r1, r0)), 2.f));
 	float t2121 = v3_dot(r0, r1) / v3_norm(r1);
 	float t2122 = -v3_dot(r0, r2) / v3_norm(r2);
-	float t221 = 3.0f / v3_norm(r0);
+	float t221 = (ORC_MUT(19) ? 2.0f : 3.0f) / v3_norm(r0);
 	float t2221 = v3_norm(v3_cross(r0, r1)) / v3_norm(r1);
 	float t2222 = -v3_norm(v3_cross(r0, r1)) / v3_norm(r2);
 	float t222 = t2221 + t2222;

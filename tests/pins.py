@@ -354,6 +354,43 @@ def f3d_dvort_structure(o):
     return f
 
 
+def f3d_dvort_second_statement(o):
+    """F3D stretching against the reference's second statement of its formula.
+
+    The reference states the filament stretching twice: src/F3D.cpp:56-85 (the
+    CPU path the oracle restates) and its OpenCL kernel
+    src/nbody.cl:686-716.  No textbook form exists; the structure checks above
+    pin the rotation part through the velocity, and this check pins the
+    isotropic part's magnitude B = 3 Gamma / (4 pi |r0|) |r0 x r1|
+    (1/|r1| - 1/|r2|) to the OpenCL statement, evaluated here in FP64.
+    """
+    f = []
+    rng = np.random.default_rng(19)
+    for _ in range(12):
+        a, b, x = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3), rng.uniform(-2, 2, 3)
+        gam, al = rng.normal(), rng.normal(size=3)
+        fil = make_f3d([a], [b], [gam])[0]
+        p = make_p3d([x], [al], [0.1])[0]
+        got = o.F3D_S2S_dvort(fil, p).astype(np.float64)
+        aa, bb = fil["start"].astype(np.float64), fil["end"].astype(np.float64)
+        xx, al64 = p["coord"].astype(np.float64), p["vorticity"].astype(np.float64)
+        r1, r2 = xx - aa, xx - bb
+        r0 = r1 - r2
+        t1 = float(fil["strength"]) / (4.0 * np.pi)
+        cl = np.linalg.norm(np.cross(r1, r0)) ** 2
+        t211 = -r0 / cl
+        t212 = r0 @ r1 / np.linalg.norm(r1) - r0 @ r2 / np.linalg.norm(r2)
+        t221 = 3.0 / np.linalg.norm(r0)
+        c01 = np.linalg.norm(np.cross(r0, r1))
+        t222 = c01 / np.linalg.norm(r1) - c01 / np.linalg.norm(r2)
+        A = t211 * t1 * t212
+        B = t221 * t1 * t222
+        want = B * al64 + np.cross(A, al64)
+        if np.linalg.norm(got - want) > 1e-4 * np.linalg.norm(want):
+            f.append(f"F3D dvort {got} != second statement {want}")
+    return f
+
+
 # ---------------------------------------------------------------------------
 # K. 2D point vortex: |u| = Gamma g_2D / (2 pi r), perpendicular to d, with
 #    the reference's orientation u = Gamma/(2 pi r^2) (d_y, -d_x)
@@ -639,6 +676,6 @@ def redistribution_moments(o):
 
 ALL_CHECKS = (kat_vortfunc, moments_3d, g_cumulative_zeta, kernels_2d, particle_invariants,
               biot_savart_pair, dvort_transpose_fd, pse_conservation, f3d_vel_quadrature,
-              f3d_dvort_structure, p2d_point_vortex, ring_centre, refcpu_vs_fp64,
+              f3d_dvort_structure, f3d_dvort_second_statement, p2d_point_vortex, ring_centre, refcpu_vs_fp64,
               vort_box_cutoff, relaxation_invariants, influence_matrix_rows,
               redistribution_moments)

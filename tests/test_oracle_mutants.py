@@ -36,6 +36,7 @@ EXPECTED = {
     16: "influence_matrix_rows",  # influence matrix stored column-major
     17: "redistribution_moments",  # m4p inner coefficient 2.5 -> 2.0
     18: "redistribution_moments",  # removed strength not redistributed
+    19: "f3d_dvort_second_statement",  # F3D dvort isotropic coefficient 3 -> 2
 }
 
 
