@@ -162,7 +162,9 @@ __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
         for (int a = 0; a < NA; ++a) acc[k][a] = 0.f;
     }
     int jv = in_vgpr(0);  // LDS index in a VGPR (no SGPR operand per source)
-#pragma unroll 4
+    // (the Gaussian's pinned branch cannot be unrolled around)
+    constexpr int kUnroll = REG == REG_GAUSSIAN ? 1 : 4;
+#pragma unroll kUnroll
     for (int j = 0; j < cnt; ++j, ++jv) {
       const float4 s = s_src[jv];
 #pragma unroll

@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_select.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
 
@@ -219,12 +220,54 @@ __global__ __launch_bounds__(256) void rd_contrib(const float4* __restrict__ pv,
   vals[e] = v;
 }
 
-__global__ __launch_bounds__(256) void rd_heads(const unsigned long long* __restrict__ keys,
-                                                long n, unsigned char* __restrict__ head) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+// Node starts: the heads of the runs of equal keys among the sorted entries
+// (sentinels excluded), in increasing order.  Heads are sparse (one per
+// node, a few per thousand entries), so they are found without a flag array
+// or a general selection: rd_head_counts counts the heads of each chunk of
+// kChunk entries, an inclusive scan of the counts gives each chunk's end
+// offset, and rd_head_write writes the chunk's head positions from its
+// start offset, a wave ballot giving each head its rank.
+constexpr int kChunk = 4096;  // entries per chunk (one 256-lane block, 16 rounds)
+
+__device__ __forceinline__ bool run_head(const unsigned long long* __restrict__ keys, long i,
+                                         long n) {
+  if (i >= n) return false;
   const unsigned long long k = keys[i];
-  head[i] = (k != kSentinel && (i == 0 || keys[i - 1] != k)) ? 1 : 0;
+  return k != kSentinel && (i == 0 || keys[i - 1] != k);
+}
+
+__global__ __launch_bounds__(256) void rd_head_counts(const unsigned long long* __restrict__ keys,
+                                                      long n, int* __restrict__ counts) {
+  __shared__ int s_w[4];
+  const long base = (long)blockIdx.x * kChunk;
+  int c = 0;
+  for (int r = 0; r < kChunk / 256; ++r) c += run_head(keys, base + r * 256 + threadIdx.x, n);
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) counts[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+__global__ __launch_bounds__(256) void rd_head_write(const unsigned long long* __restrict__ keys,
+                                                     long n, const int* __restrict__ incl,
+                                                     long* __restrict__ starts) {
+  __shared__ int s_w[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long base = (long)blockIdx.x * kChunk;
+  int off = blockIdx.x > 0 ? incl[blockIdx.x - 1] : 0;
+  for (int r = 0; r < kChunk / 256; ++r) {
+    const long i = base + r * 256 + threadIdx.x;
+    const bool h = run_head(keys, i, n);
+    const unsigned long long m = __ballot(h);
+    const int rank = __popcll(m & ((1ull << lane) - 1ull));
+    __syncthreads();  // s_w of the previous round has been read
+    if (lane == 0) s_w[w] = __popcll(m);
+    __syncthreads();
+    int wo = 0;
+    for (int q = 0; q < w; ++q) wo += s_w[q];
+    if (h) starts[off + wo + rank] = i;
+    off += s_w[0] + s_w[1] + s_w[2] + s_w[3];
+  }
 }
 
 // One wave per node: the wave reads the node's run of entries 64 at a time
@@ -503,26 +546,19 @@ static int redistribute_dim(Runtime& rt, Device& d, const RedistJob& j, float** 
   const long WD = DIM == 3 ? (long)W * W * W : (long)W * W;
   const long ne = (long)n * WD;
   if (ne >= 2147483647L) return fail("too many grid contributions for one call", hipSuccess);
-  // ---- workspace ----
-  size_t sort_tmp = 0, sel_tmp = 0;
+  // ---- workspace: grid entries (sized by the entry count) ----
+  const long nc = (ne + kChunk - 1) / kChunk;  // chunks of the node-start search
+  size_t sort_tmp = 0, scan_tmp = 0;
   e = rocprim::radix_sort_pairs(nullptr, sort_tmp, (unsigned long long*)nullptr,
                                 (unsigned long long*)nullptr, (float4*)nullptr, (float4*)nullptr,
                                 (size_t)ne, 0, 64, s);
   if (e != hipSuccess) return fail("radix sort size", e);
-  e = rocprim::select(nullptr, sel_tmp, rocprim::counting_iterator<long>(0),
-                      (unsigned char*)nullptr, (long*)nullptr, (long*)nullptr, (size_t)ne, s);
-  if (e != hipSuccess) return fail("select size", e);
-  size_t sel_tmp2 = 0;
-  e = rocprim::select(nullptr, sel_tmp2, rocprim::counting_iterator<int>(0), (unsigned char*)nullptr,
-                      (int*)nullptr, (int*)nullptr, (size_t)ne, s);
-  if (e != hipSuccess) return fail("select size", e);
-  const size_t tmp = std::max(sort_tmp, std::max(sel_tmp, sel_tmp2));
+  e = rocprim::inclusive_scan(nullptr, scan_tmp, (int*)nullptr, (int*)nullptr, (size_t)nc,
+                              rocprim::plus<int>(), s);
+  if (e != hipSuccess) return fail("scan size", e);
+  const size_t tmp = std::max(sort_tmp, scan_tmp);
   const size_t need = al256(sizeof(unsigned long long) * ne) * 2 + al256(sizeof(float4) * ne) * 2 +
-                      al256(tmp) + al256(ne) + al256(sizeof(long) * ne) + al256(sizeof(long)) +
-                      al256(sizeof(float4) * ne) * 4 + al256(sizeof(float) * ne) * 2 +
-                      al256(sizeof(int) * ne) + al256(sizeof(int)) + al256(sizeof(double) * kPartBlocks * 3) +
-                      al256(sizeof(float) * kPartBlocks) * 2 + al256(sizeof(int) * 1024) +
-                      al256(sizeof(float) * 7 * ne);
+                      al256(tmp) + al256(sizeof(int) * nc) * 2;
   Arena ar;
   ar.base = (char*)device_buffer(rt, d, SLOT_RD, need);
   ar.cap = need;
@@ -532,47 +568,65 @@ static int redistribute_dim(Runtime& rt, Device& d, const RedistJob& j, float** 
   auto* v_in = ar.take<float4>(ne);
   auto* v_out = ar.take<float4>(ne);
   void* t = ar.take<char>(tmp);
-  auto* head = ar.take<unsigned char>(ne);
-  auto* starts = ar.take<long>(ne);
-  auto* nstart = ar.take<long>(1);
-  auto* pos0 = ar.take<float4>(ne);
-  auto* w0 = ar.take<float4>(ne);
-  auto* pos1 = ar.take<float4>(ne);
-  auto* w1 = ar.take<float4>(ne);
-  auto* str0 = ar.take<float>(ne);
-  auto* str1 = ar.take<float>(ne);
-  auto* kidx = ar.take<int>(ne);
-  auto* nkept = ar.take<int>(1);
-  auto* pdef = ar.take<double>(kPartBlocks * 3);
-  auto* psmin = ar.take<float>(kPartBlocks);
-  auto* psmax = ar.take<float>(kPartBlocks);
-  auto* counts = ar.take<int>(1024);
-  auto* packed = ar.take<float>(7 * (size_t)ne);
-  if (!packed) return fail("redistribution workspace", hipSuccess);
-  // ---- contributions, sort, node sums ----
+  auto* ccount = ar.take<int>(nc);
+  auto* cincl = ar.take<int>(nc);
+  if (!cincl) return fail("redistribution workspace", hipSuccess);
+  // ---- contributions, sort, node starts ----
   e = launch_contrib<DIM>(j.rid, j.pv, j.a, ne, g, k_in, v_in, s);
   if (e != hipSuccess) return fail("redistribution contributions", e);
   size_t tb = tmp;
   e = rocprim::radix_sort_pairs(t, tb, k_in, k_out, v_in, v_out, (size_t)ne, 0, key_bits, s);
   if (e != hipSuccess) return fail("redistribution sort", e);
-  hipLaunchKernelGGL(rd_heads, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, k_out, ne, head);
+  hipLaunchKernelGGL(rd_head_counts, dim3((unsigned)nc), dim3(256), 0, s, k_out, ne, ccount);
   tb = tmp;
-  e = rocprim::select(t, tb, rocprim::counting_iterator<long>(0), head, starts, nstart, (size_t)ne, s);
-  if (e != hipSuccess) return fail("redistribution node starts", e);
-  long hn = 0;
-  e = hipMemcpyAsync(&hn, nstart, sizeof(long), hipMemcpyDeviceToHost, s);
+  e = rocprim::inclusive_scan(t, tb, ccount, cincl, (size_t)nc, rocprim::plus<int>(), s);
+  if (e != hipSuccess) return fail("redistribution node count", e);
+  int hn = 0;
+  e = hipMemcpyAsync(&hn, cincl + (nc - 1), sizeof(int), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return fail("redistribution node count", e);
-  int nn = (int)hn;
+  int nn = hn;
   if (nn == 0) return 0;
+  // ---- workspace: nodes (sized by the node count) ----
+  size_t sel_tmp = 0;
+  e = rocprim::select(nullptr, sel_tmp, rocprim::counting_iterator<int>(0), (unsigned char*)nullptr,
+                      (int*)nullptr, (int*)nullptr, (size_t)nn, s);
+  if (e != hipSuccess) return fail("select size", e);
+  const size_t nneed = al256(sel_tmp) + al256(nn) + al256(sizeof(long) * nn) +
+                       al256(sizeof(float4) * nn) * 4 + al256(sizeof(float) * nn) * 2 +
+                       al256(sizeof(int) * nn) + al256(sizeof(int)) +
+                       al256(sizeof(double) * kPartBlocks * 3) + al256(sizeof(float) * kPartBlocks) * 2 +
+                       al256(sizeof(int) * 1024) + al256(sizeof(float) * 7 * (size_t)nn);
+  Arena na;
+  na.base = (char*)device_buffer(rt, d, SLOT_RD_NODES, nneed);
+  na.cap = nneed;
+  if (!na.base) return -1;
+  void* t2 = na.take<char>(sel_tmp);
+  auto* head = na.take<unsigned char>(nn);
+  auto* starts = na.take<long>(nn);
+  auto* pos0 = na.take<float4>(nn);
+  auto* w0 = na.take<float4>(nn);
+  auto* pos1 = na.take<float4>(nn);
+  auto* w1 = na.take<float4>(nn);
+  auto* str0 = na.take<float>(nn);
+  auto* str1 = na.take<float>(nn);
+  auto* kidx = na.take<int>(nn);
+  auto* nkept = na.take<int>(1);
+  auto* pdef = na.take<double>(kPartBlocks * 3);
+  auto* psmin = na.take<float>(kPartBlocks);
+  auto* psmax = na.take<float>(kPartBlocks);
+  auto* counts = na.take<int>(1024);
+  auto* packed = na.take<float>(7 * (size_t)nn);
+  if (!packed) return fail("redistribution workspace", hipSuccess);
+  hipLaunchKernelGGL(rd_head_write, dim3((unsigned)nc), dim3(256), 0, s, k_out, ne, cincl, starts);
   hipLaunchKernelGGL((rd_nodes<DIM>), dim3((nn + 3) / 4), dim3(256), 0, s, k_out, v_out, ne,
                      starts, nn, g, pos0, w0, str0);
   // ---- removal: strength <= threshold and index >= max_keep ----
   auto remove = [&](float thr, int max_keep) -> int {
     hipLaunchKernelGGL(rd_flags, dim3(kPartBlocks), dim3(256), 0, s, str0, w0, nn, thr, max_keep,
                        head, pdef);
-    size_t tb2 = tmp;
-    hipError_t ee = rocprim::select(t, tb2, rocprim::counting_iterator<int>(0), head, kidx, nkept,
+    size_t tb2 = sel_tmp;
+    hipError_t ee = rocprim::select(t2, tb2, rocprim::counting_iterator<int>(0), head, kidx, nkept,
                                     (size_t)nn, s);
     if (ee != hipSuccess) return fail("redistribution compaction", ee);
     std::vector<double> hdef(kPartBlocks * 3);

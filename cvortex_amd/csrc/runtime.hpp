@@ -31,7 +31,8 @@ enum Slot : int {
   SLOT_ORDER,  // target-order workspace
   SLOT_PERM,   // target permutation
   SLOT_PERM2,  // source permutation (vorticity field)
-  SLOT_RD,     // redistribution workspace
+  SLOT_RD,     // redistribution workspace: grid entries
+  SLOT_RD_NODES,  // redistribution workspace: nodes
   SLOT_RD_SMALL,
   SLOT_COUNT
 };
