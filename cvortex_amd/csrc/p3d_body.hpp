@@ -255,7 +255,10 @@ __device__ __forceinline__ void p3d_block(const P3DKParams& p, const RegConsts& 
     // the LDS index runs in a VGPR too: an SGPR index would cost an
     // SGPR-reading move per source to form the ds_read address
     int jv = in_vgpr(0);
-#pragma unroll 2
+    // two sources per iteration, except around the velocity-only Gaussian's
+    // pinned near-range branch, which cannot be unrolled
+    constexpr int kUnroll = (R::SPLIT && (MASK & (OUT_DVORT | OUT_VISC)) == 0) ? 1 : 2;
+#pragma unroll kUnroll
     for (int j = 0; j < cnt; ++j, ++jv) {
       const float4 sp = sm.pv[jv];
       const float4 sa = sm.a[jv];

@@ -48,7 +48,6 @@ namespace cvtx {
 namespace {
 
 constexpr int kPartBlocks = 512;  // fixed grid of the reductions (deterministic)
-constexpr unsigned long long kSentinel = ~0ull;
 
 // Redistribution functions, reference src/RedistFunc.cpp:36-96, in FP32
 // with the reference's operation order.
@@ -165,16 +164,15 @@ __global__ __launch_bounds__(256) void rd_bounds(const float4* __restrict__ pv, 
     }
 }
 
-// One thread per (particle, neighbour): entry e = i * W^DIM + j.
-// 3D: pv = {x, y, z, -}, a = {alpha, -}; 2D: pv = {x, y, alpha, -}.
+// Entry e = i * W^DIM + j: the contribution of particle i to its j-th
+// neighbour node (a window of W = 2R + 1 nodes per axis around the nearest
+// node).  3D: pv = {x, y, z, -}, a = {alpha, -}; 2D: pv = {x, y, alpha, -}.
+// Returns false for an exactly-zero contribution (the reference makes no
+// node for it).
 template <int DIM, int RID>
-__global__ __launch_bounds__(256) void rd_contrib(const float4* __restrict__ pv,
-                                                  const float4* __restrict__ a, long n_entries,
-                                                  Grid g, unsigned long long* __restrict__ keys,
-                                                  float4* __restrict__ vals) {
-  // n_entries < 2^31 (checked on the host): 32-bit indices
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= n_entries) return;
+__device__ __forceinline__ bool rd_entry(const float4* __restrict__ pv,
+                                         const float4* __restrict__ a, int e, const Grid& g,
+                                         unsigned long long& key, float4& v) {
   const int W = 2 * g.R + 1;
   const int WD = DIM == 3 ? W * W * W : W * W;
   const int i = e / WD;
@@ -205,35 +203,96 @@ __global__ __launch_bounds__(256) void rd_contrib(const float4* __restrict__ pv,
     const float U = fabsf(rn_mul(rn_sub(x[c], node), g.rh));
     frac = c == 0 ? rd_f<RID>(U) : rn_mul(frac, rd_f<RID>(U));
   }
-  float4 v;
-  unsigned long long key;
   if constexpr (DIM == 3) {
     const float4 al = a[i];
     v = make_float4(rn_mul(al.x, frac), rn_mul(al.y, frac), rn_mul(al.z, frac), 0.f);
-    const bool zero = v.x == 0.f && v.y == 0.f && v.z == 0.f;
-    key = zero ? kSentinel : (spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2));
+    key = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    return !(v.x == 0.f && v.y == 0.f && v.z == 0.f);
   } else {
     v = make_float4(rn_mul(p.z, frac), 0.f, 0.f, 0.f);
-    key = v.x == 0.f ? kSentinel : (spread2(q[0]) | (spread2(q[1]) << 1));
+    key = spread2(q[0]) | (spread2(q[1]) << 1);
+    return v.x != 0.f;
   }
-  keys[e] = key;
-  vals[e] = v;
 }
 
-// Node starts: the heads of the runs of equal keys among the sorted entries
-// (sentinels excluded), in increasing order.  Heads are sparse (one per
-// node, a few per thousand entries), so they are found without a flag array
-// or a general selection: rd_head_counts counts the heads of each chunk of
-// kChunk entries, an inclusive scan of the counts gives each chunk's end
-// offset, and rd_head_write writes the chunk's head positions from its
-// start offset, a wave ballot giving each head its rank.
-constexpr int kChunk = 4096;  // entries per chunk (one 256-lane block, 16 rounds)
+// Chunked stable compaction (used for the nonzero entries and for the node
+// starts): a first kernel counts the selected items of each chunk of kChunk
+// = 16 rounds of 256, an inclusive scan of the counts gives each chunk's end
+// offset, and a second kernel writes the chunk's items from its start
+// offset in their original order, a wave ballot ranking them within a round.
+constexpr int kChunk = 4096;
 
+__device__ __forceinline__ int block_sum(int c, int* s_w) {
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
+  __syncthreads();
+  return s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// Within one round of a chunk: the output slot of this lane's item (if it is
+// selected), given the round's first slot; advances `off` past the round.
+__device__ __forceinline__ int round_slot(bool sel, int& off, int* s_w) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(sel);
+  const int rank = __popcll(m & ((1ull << lane) - 1ull));
+  __syncthreads();  // s_w of the previous round has been read
+  if (lane == 0) s_w[w] = __popcll(m);
+  __syncthreads();
+  int wo = 0;
+  for (int q = 0; q < w; ++q) wo += s_w[q];
+  const int slot = off + wo + rank;
+  off += s_w[0] + s_w[1] + s_w[2] + s_w[3];
+  return slot;
+}
+
+template <int DIM, int RID>
+__global__ __launch_bounds__(256) void rd_contrib_count(const float4* __restrict__ pv,
+                                                        const float4* __restrict__ a, long ne,
+                                                        Grid g, int* __restrict__ counts) {
+  __shared__ int s_w[4];
+  const long base = (long)blockIdx.x * kChunk;
+  int c = 0;
+  for (int r = 0; r < kChunk / 256; ++r) {
+    const long e = base + r * 256 + threadIdx.x;
+    unsigned long long key;
+    float4 v;
+    if (e < ne && rd_entry<DIM, RID>(pv, a, (int)e, g, key, v)) ++c;
+  }
+  c = block_sum(c, s_w);
+  if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+// The nonzero entries, compacted in entry order (ne < 2^31, checked on the
+// host: 32-bit entry indices).
+template <int DIM, int RID>
+__global__ __launch_bounds__(256) void rd_contrib_write(const float4* __restrict__ pv,
+                                                        const float4* __restrict__ a, long ne,
+                                                        Grid g, const int* __restrict__ incl,
+                                                        unsigned long long* __restrict__ keys,
+                                                        float4* __restrict__ vals) {
+  __shared__ int s_w[4];
+  const long base = (long)blockIdx.x * kChunk;
+  int off = blockIdx.x > 0 ? incl[blockIdx.x - 1] : 0;
+  for (int r = 0; r < kChunk / 256; ++r) {
+    const long e = base + r * 256 + threadIdx.x;
+    unsigned long long key = 0;
+    float4 v;
+    const bool sel = e < ne && rd_entry<DIM, RID>(pv, a, (int)e, g, key, v);
+    const int slot = round_slot(sel, off, s_w);
+    if (sel) {
+      keys[slot] = key;
+      vals[slot] = v;
+    }
+  }
+}
+
+// Node starts: the heads of the runs of equal keys among the sorted nonzero
+// entries, in increasing order, by the chunked compaction above.  Heads are
+// sparse (one per node, a few per hundred entries), so no flag array is
+// written.
 __device__ __forceinline__ bool run_head(const unsigned long long* __restrict__ keys, long i,
                                          long n) {
-  if (i >= n) return false;
-  const unsigned long long k = keys[i];
-  return k != kSentinel && (i == 0 || keys[i - 1] != k);
+  return i < n && (i == 0 || keys[i - 1] != keys[i]);
 }
 
 __global__ __launch_bounds__(256) void rd_head_counts(const unsigned long long* __restrict__ keys,
@@ -242,31 +301,21 @@ __global__ __launch_bounds__(256) void rd_head_counts(const unsigned long long* 
   const long base = (long)blockIdx.x * kChunk;
   int c = 0;
   for (int r = 0; r < kChunk / 256; ++r) c += run_head(keys, base + r * 256 + threadIdx.x, n);
-  for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
-  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) counts[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+  c = block_sum(c, s_w);
+  if (threadIdx.x == 0) counts[blockIdx.x] = c;
 }
 
 __global__ __launch_bounds__(256) void rd_head_write(const unsigned long long* __restrict__ keys,
                                                      long n, const int* __restrict__ incl,
                                                      long* __restrict__ starts) {
   __shared__ int s_w[4];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const long base = (long)blockIdx.x * kChunk;
   int off = blockIdx.x > 0 ? incl[blockIdx.x - 1] : 0;
   for (int r = 0; r < kChunk / 256; ++r) {
     const long i = base + r * 256 + threadIdx.x;
     const bool h = run_head(keys, i, n);
-    const unsigned long long m = __ballot(h);
-    const int rank = __popcll(m & ((1ull << lane) - 1ull));
-    __syncthreads();  // s_w of the previous round has been read
-    if (lane == 0) s_w[w] = __popcll(m);
-    __syncthreads();
-    int wo = 0;
-    for (int q = 0; q < w; ++q) wo += s_w[q];
-    if (h) starts[off + wo + rank] = i;
-    off += s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    const int slot = round_slot(h, off, s_w);
+    if (h) starts[slot] = i;
   }
 }
 
@@ -459,15 +508,30 @@ struct Arena {
 };
 
 template <int DIM>
-hipError_t launch_contrib(int rid, const float4* pv, const float4* a, long ne, const Grid& g,
-                          unsigned long long* keys, float4* vals, hipStream_t s) {
-  const dim3 grid((unsigned)((ne + 255) / 256));
+hipError_t launch_contrib_count(int rid, const float4* pv, const float4* a, long ne, const Grid& g,
+                                int* counts, hipStream_t s) {
+  const dim3 grid((unsigned)((ne + kChunk - 1) / kChunk));
   switch (rid) {
-    case 0: hipLaunchKernelGGL((rd_contrib<DIM, 0>), grid, dim3(256), 0, s, pv, a, ne, g, keys, vals); break;
-    case 1: hipLaunchKernelGGL((rd_contrib<DIM, 1>), grid, dim3(256), 0, s, pv, a, ne, g, keys, vals); break;
-    case 2: hipLaunchKernelGGL((rd_contrib<DIM, 2>), grid, dim3(256), 0, s, pv, a, ne, g, keys, vals); break;
-    case 3: hipLaunchKernelGGL((rd_contrib<DIM, 3>), grid, dim3(256), 0, s, pv, a, ne, g, keys, vals); break;
-    default: hipLaunchKernelGGL((rd_contrib<DIM, 4>), grid, dim3(256), 0, s, pv, a, ne, g, keys, vals); break;
+    case 0: hipLaunchKernelGGL((rd_contrib_count<DIM, 0>), grid, dim3(256), 0, s, pv, a, ne, g, counts); break;
+    case 1: hipLaunchKernelGGL((rd_contrib_count<DIM, 1>), grid, dim3(256), 0, s, pv, a, ne, g, counts); break;
+    case 2: hipLaunchKernelGGL((rd_contrib_count<DIM, 2>), grid, dim3(256), 0, s, pv, a, ne, g, counts); break;
+    case 3: hipLaunchKernelGGL((rd_contrib_count<DIM, 3>), grid, dim3(256), 0, s, pv, a, ne, g, counts); break;
+    default: hipLaunchKernelGGL((rd_contrib_count<DIM, 4>), grid, dim3(256), 0, s, pv, a, ne, g, counts); break;
+  }
+  return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_contrib_write(int rid, const float4* pv, const float4* a, long ne, const Grid& g,
+                                const int* incl, unsigned long long* keys, float4* vals,
+                                hipStream_t s) {
+  const dim3 grid((unsigned)((ne + kChunk - 1) / kChunk));
+  switch (rid) {
+    case 0: hipLaunchKernelGGL((rd_contrib_write<DIM, 0>), grid, dim3(256), 0, s, pv, a, ne, g, incl, keys, vals); break;
+    case 1: hipLaunchKernelGGL((rd_contrib_write<DIM, 1>), grid, dim3(256), 0, s, pv, a, ne, g, incl, keys, vals); break;
+    case 2: hipLaunchKernelGGL((rd_contrib_write<DIM, 2>), grid, dim3(256), 0, s, pv, a, ne, g, incl, keys, vals); break;
+    case 3: hipLaunchKernelGGL((rd_contrib_write<DIM, 3>), grid, dim3(256), 0, s, pv, a, ne, g, incl, keys, vals); break;
+    default: hipLaunchKernelGGL((rd_contrib_write<DIM, 4>), grid, dim3(256), 0, s, pv, a, ne, g, incl, keys, vals); break;
   }
   return hipGetLastError();
 }
@@ -539,53 +603,73 @@ static int redistribute_dim(Runtime& rt, Device& d, const RedistJob& j, float** 
       return fail("grid too fine for the particles' extent", hipSuccess);
     while (axis_bits < 32 && (double)(1ull << axis_bits) <= span) ++axis_bits;
   }
-  // the sort only needs the interleaved bits of valid keys plus one more,
-  // above which every valid key is zero and the sentinel is not
-  const int key_bits = DIM * axis_bits + 1 < 64 ? DIM * axis_bits + 1 : 64;
+  // the sort only needs the interleaved bits of the node indices
+  const int key_bits = DIM * axis_bits < 64 ? DIM * axis_bits : 64;
   const int W = 2 * g.R + 1;
   const long WD = DIM == 3 ? (long)W * W * W : (long)W * W;
   const long ne = (long)n * WD;
   if (ne >= 2147483647L) return fail("too many grid contributions for one call", hipSuccess);
-  // ---- workspace: grid entries (sized by the entry count) ----
-  const long nc = (ne + kChunk - 1) / kChunk;  // chunks of the node-start search
-  size_t sort_tmp = 0, scan_tmp = 0;
-  e = rocprim::radix_sort_pairs(nullptr, sort_tmp, (unsigned long long*)nullptr,
-                                (unsigned long long*)nullptr, (float4*)nullptr, (float4*)nullptr,
-                                (size_t)ne, 0, 64, s);
-  if (e != hipSuccess) return fail("radix sort size", e);
+  // ---- nonzero entries: count per chunk, scan, compacting write ----
+  const long nc = (ne + kChunk - 1) / kChunk;
+  size_t scan_tmp = 0;
   e = rocprim::inclusive_scan(nullptr, scan_tmp, (int*)nullptr, (int*)nullptr, (size_t)nc,
                               rocprim::plus<int>(), s);
   if (e != hipSuccess) return fail("scan size", e);
-  const size_t tmp = std::max(sort_tmp, scan_tmp);
-  const size_t need = al256(sizeof(unsigned long long) * ne) * 2 + al256(sizeof(float4) * ne) * 2 +
-                      al256(tmp) + al256(sizeof(int) * nc) * 2;
+  Arena ca;
+  const size_t cneed = al256(sizeof(int) * nc) * 2 + al256(scan_tmp);
+  ca.base = (char*)device_buffer(rt, d, SLOT_RD_CHUNKS, cneed);
+  ca.cap = cneed;
+  if (!ca.base) return -1;
+  auto* ccount = ca.take<int>(nc);
+  auto* cincl = ca.take<int>(nc);
+  void* ct = ca.take<char>(scan_tmp);
+  // the count of the chunks' items (in cincl's last element) on the host
+  auto scan_total = [&](long chunks, int* total) -> bool {
+    size_t tb = scan_tmp;
+    hipError_t ee = rocprim::inclusive_scan(ct, tb, ccount, cincl, (size_t)chunks,
+                                            rocprim::plus<int>(), s);
+    if (ee == hipSuccess)
+      ee = hipMemcpyAsync(total, cincl + (chunks - 1), sizeof(int), hipMemcpyDeviceToHost, s);
+    if (ee == hipSuccess) ee = hipStreamSynchronize(s);
+    if (ee != hipSuccess) {
+      fail("redistribution scan", ee);
+      return false;
+    }
+    return true;
+  };
+  e = launch_contrib_count<DIM>(j.rid, j.pv, j.a, ne, g, ccount, s);
+  if (e != hipSuccess) return fail("redistribution contributions", e);
+  int nz = 0;
+  if (!scan_total(nc, &nz)) return -1;
+  if (nz == 0) return 0;
+  // ---- workspace: the nonzero entries and the sort ----
+  size_t sort_tmp = 0;
+  e = rocprim::radix_sort_pairs(nullptr, sort_tmp, (unsigned long long*)nullptr,
+                                (unsigned long long*)nullptr, (float4*)nullptr, (float4*)nullptr,
+                                (size_t)nz, 0, 64, s);
+  if (e != hipSuccess) return fail("radix sort size", e);
+  const size_t need = al256(sizeof(unsigned long long) * nz) * 2 + al256(sizeof(float4) * nz) * 2 +
+                      al256(sort_tmp);
   Arena ar;
   ar.base = (char*)device_buffer(rt, d, SLOT_RD, need);
   ar.cap = need;
   if (!ar.base) return -1;
-  auto* k_in = ar.take<unsigned long long>(ne);
-  auto* k_out = ar.take<unsigned long long>(ne);
-  auto* v_in = ar.take<float4>(ne);
-  auto* v_out = ar.take<float4>(ne);
-  void* t = ar.take<char>(tmp);
-  auto* ccount = ar.take<int>(nc);
-  auto* cincl = ar.take<int>(nc);
-  if (!cincl) return fail("redistribution workspace", hipSuccess);
-  // ---- contributions, sort, node starts ----
-  e = launch_contrib<DIM>(j.rid, j.pv, j.a, ne, g, k_in, v_in, s);
+  auto* k_in = ar.take<unsigned long long>(nz);
+  auto* k_out = ar.take<unsigned long long>(nz);
+  auto* v_in = ar.take<float4>(nz);
+  auto* v_out = ar.take<float4>(nz);
+  void* t = ar.take<char>(sort_tmp);
+  if (!t) return fail("redistribution workspace", hipSuccess);
+  e = launch_contrib_write<DIM>(j.rid, j.pv, j.a, ne, g, cincl, k_in, v_in, s);
   if (e != hipSuccess) return fail("redistribution contributions", e);
-  size_t tb = tmp;
-  e = rocprim::radix_sort_pairs(t, tb, k_in, k_out, v_in, v_out, (size_t)ne, 0, key_bits, s);
+  size_t tb = sort_tmp;
+  e = rocprim::radix_sort_pairs(t, tb, k_in, k_out, v_in, v_out, (size_t)nz, 0, key_bits, s);
   if (e != hipSuccess) return fail("redistribution sort", e);
-  hipLaunchKernelGGL(rd_head_counts, dim3((unsigned)nc), dim3(256), 0, s, k_out, ne, ccount);
-  tb = tmp;
-  e = rocprim::inclusive_scan(t, tb, ccount, cincl, (size_t)nc, rocprim::plus<int>(), s);
-  if (e != hipSuccess) return fail("redistribution node count", e);
-  int hn = 0;
-  e = hipMemcpyAsync(&hn, cincl + (nc - 1), sizeof(int), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) return fail("redistribution node count", e);
-  int nn = hn;
+  // ---- node starts ----
+  const long nh = (nz + kChunk - 1) / kChunk;
+  hipLaunchKernelGGL(rd_head_counts, dim3((unsigned)nh), dim3(256), 0, s, k_out, (long)nz, ccount);
+  int nn = 0;
+  if (!scan_total(nh, &nn)) return -1;
   if (nn == 0) return 0;
   // ---- workspace: nodes (sized by the node count) ----
   size_t sel_tmp = 0;
@@ -618,8 +702,9 @@ static int redistribute_dim(Runtime& rt, Device& d, const RedistJob& j, float** 
   auto* counts = na.take<int>(1024);
   auto* packed = na.take<float>(7 * (size_t)nn);
   if (!packed) return fail("redistribution workspace", hipSuccess);
-  hipLaunchKernelGGL(rd_head_write, dim3((unsigned)nc), dim3(256), 0, s, k_out, ne, cincl, starts);
-  hipLaunchKernelGGL((rd_nodes<DIM>), dim3((nn + 3) / 4), dim3(256), 0, s, k_out, v_out, ne,
+  hipLaunchKernelGGL(rd_head_write, dim3((unsigned)nh), dim3(256), 0, s, k_out, (long)nz, cincl,
+                     starts);
+  hipLaunchKernelGGL((rd_nodes<DIM>), dim3((nn + 3) / 4), dim3(256), 0, s, k_out, v_out, (long)nz,
                      starts, nn, g, pos0, w0, str0);
   // ---- removal: strength <= threshold and index >= max_keep ----
   auto remove = [&](float thr, int max_keep) -> int {

@@ -33,6 +33,7 @@ enum Slot : int {
   SLOT_PERM2,  // source permutation (vorticity field)
   SLOT_RD,     // redistribution workspace: grid entries
   SLOT_RD_NODES,  // redistribution workspace: nodes
+  SLOT_RD_CHUNKS,  // redistribution workspace: per-chunk counts and scan
   SLOT_RD_SMALL,
   SLOT_COUNT
 };
