@@ -9,6 +9,7 @@
 // per line: "kind reg mask acc bucket_src bucket_tgt T n_split ms".
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -66,19 +67,27 @@ int size_bucket(int n) {
 LaunchCfg choose_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt, int n_cu) {
   auto it = cache().find(Key{kind, reg, mask, acc, size_bucket(n_src), size_bucket(n_tgt)});
   if (it != cache().end()) return shape(it->second.T, it->second.n_split, n_src);
+  const int cus = n_cu > 0 ? n_cu : 256;
+  // Spatially ordered targets (Gaussian P3D, compensated sums).  One target
+  // per lane: a 64-target wave of Morton-ordered targets is more compact
+  // than a 128-target one, so fewer of its source rounds take the FP64
+  // near-field branch.  And enough source partitions for about 64
+  // workgroups per CU: the near-field work is concentrated in the workgroups
+  // whose targets sit in dense regions, and partitioning spreads each of
+  // them over several CUs instead of leaving them as the tail.  Measured on
+  // the ring (vel+dvort, ms; default before -> this shape):
+  //   2^14: 0.29 -> 0.26,  2^16: 3.73 -> 2.86,  2^18: 58.0 -> 43.1,
+  //   2^20: 710.6 -> 681.0,  2^22: 13742 -> 10581.
+  // The plain FP32 kernel has no FP64 branch and keeps the general shape
+  // (two targets per lane: 678 ms against 861 ms with one at 2^20).
+  if (kind == KIND_P3D && acc == ACC_COMP && p3d_wants_order(reg, n_src, n_tgt)) {
+    const long long blocks = ((long long)n_tgt + 255) / 256;
+    const long long want = 64LL * cus;
+    const long long k = blocks >= want ? 1 : (want + blocks - 1) / blocks;
+    return shape(1, (int)std::min<long long>(k, 1 << 20), n_src);
+  }
   LaunchCfg c = default_cfg(n_src, n_tgt, n_cu);
-  // Spatially ordered targets (Gaussian P3D, compensated sums): a
-  // 256-target workgroup's waves are more compact than a 512-target one's,
-  // so fewer of their pairs take the FP64 near-field branch (2^20 ring:
-  // vel+dvort 723.0 ms against 726.8 ms, +visc 747 ms against 786 ms).  The
-  // plain FP32 kernel has no such branch and loses its second target's
-  // reuse instead (861 ms against 678 ms), so it keeps T = 2.  Keep T = 1
-  // only when it still gives several workgroups per CU without source
-  // partitions.
-  const bool many_blocks = (long long)(n_tgt + 255) / 256 >= 4LL * (n_cu > 0 ? n_cu : 256);
-  if (kind == KIND_P3D && acc == ACC_COMP && p3d_wants_order(reg, n_src, n_tgt) && c.T == 2 &&
-      c.n_split == 1 && many_blocks)
-    c.T = 1;
+  const bool many_blocks = (long long)(n_tgt + 255) / 256 >= 4LL * cus;
   // Filaments with one output (vel or dvort alone): one target per lane fits
   // in 48-50 VGPRs, eight waves per SIMD against five or six with two, and
   // the extra waves pair more instructions for dual issue than the second
