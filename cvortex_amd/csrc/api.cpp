@@ -119,21 +119,7 @@ hipStream_t pick_stream(Device* d, void* s) { return s ? (hipStream_t)s : d->str
 bool make_launch_shape(int T, int n_split, int n_src, LaunchCfg* c) {
   if (T != 1 && T != 2 && T != 4) return false;
   if (n_split < 1) return false;
-  c->T = T;
-  c->wg = 256;
-  const long tiles = ((long)n_src + 255) / 256;
-  if (n_split == 1 || tiles <= 1) {
-    c->n_split = 1;
-    c->split_len = 0;
-    return true;
-  }
-  const long per = (tiles + n_split - 1) / n_split;
-  c->split_len = (int)(per * 256);
-  c->n_split = (int)((n_src + c->split_len - 1) / c->split_len);
-  if (c->n_split <= 1) {
-    c->n_split = 1;
-    c->split_len = 0;
-  }
+  *c = launch_shape(T, n_split, n_src);
   return true;
 }
 

@@ -37,14 +37,15 @@ std::map<Key, Entry>& cache() {
   return c;
 }
 
-LaunchCfg shape(int T, int n_split, int n_src) {
+}  // namespace
+
+LaunchCfg launch_shape(int T, int n_split, int n_src) {
   LaunchCfg c;
   c.T = T;
   c.wg = 256;
   const long tiles = ((long)n_src + 255) / 256;
   if (n_split <= 1 || tiles <= 1) return c;
-  const long per = (tiles + n_split - 1) / n_split;
-  c.split_len = (int)(per * 256);
+  c.split_len = (int)((tiles + n_split - 1) / n_split * 256);
   c.n_split = (int)((n_src + c.split_len - 1) / c.split_len);
   if (c.n_split <= 1) {
     c.n_split = 1;
@@ -52,8 +53,6 @@ LaunchCfg shape(int T, int n_split, int n_src) {
   }
   return c;
 }
-
-}  // namespace
 
 int size_bucket(int n) {
   int b = 0;
@@ -66,7 +65,7 @@ int size_bucket(int n) {
 
 LaunchCfg choose_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt, int n_cu) {
   auto it = cache().find(Key{kind, reg, mask, acc, size_bucket(n_src), size_bucket(n_tgt)});
-  if (it != cache().end()) return shape(it->second.T, it->second.n_split, n_src);
+  if (it != cache().end()) return launch_shape(it->second.T, it->second.n_split, n_src);
   const int cus = n_cu > 0 ? n_cu : 256;
   // Spatially ordered targets (Gaussian P3D, compensated sums).  One target
   // per lane: a 64-target wave of Morton-ordered targets is more compact
@@ -84,7 +83,7 @@ LaunchCfg choose_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt,
     const long long blocks = ((long long)n_tgt + 255) / 256;
     const long long want = 64LL * cus;
     const long long k = blocks >= want ? 1 : (want + blocks - 1) / blocks;
-    return shape(1, (int)std::min<long long>(k, 1 << 20), n_src);
+    return launch_shape(1, (int)std::min<long long>(k, 1 << 20), n_src);
   }
   LaunchCfg c = default_cfg(n_src, n_tgt, n_cu);
   const bool many_blocks = (long long)(n_tgt + 255) / 256 >= 4LL * cus;
@@ -142,7 +141,7 @@ std::vector<LaunchCfg> candidates(int n_src, int n_tgt, int n_cu) {
     for (int k = 1; k <= 256; k *= 2) {
       if (k > 1 && k > tiles) break;
       if (blocks * k > 64L * (n_cu > 0 ? n_cu : 256)) break;
-      out.push_back(shape(T, k, n_src));
+      out.push_back(launch_shape(T, k, n_src));
     }
   }
   return out;

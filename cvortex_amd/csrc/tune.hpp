@@ -14,6 +14,10 @@ LaunchCfg choose_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt,
 // Record a measured best shape for the bucket of (n_src, n_tgt).
 void record_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt, int T, int n_split);
 
+// The launch shape with T targets per lane and about n_split source
+// partitions of whole 256-source tiles.
+LaunchCfg launch_shape(int T, int n_split, int n_src);
+
 // Shape-cache bucket: floor(log2) of the sizes.
 int size_bucket(int n);
 
