@@ -259,8 +259,14 @@ def bench_cube(args, cv, D, W, o, stream):
         Ve2e = cv.P3D_M2M_vel(p, p["coord"], "winckelmans", sigma)
     e2e_ms = (time.perf_counter() - t0) / 10 * 1e3
     v64 = o.f64_P3D_vel_pts(p, p["coord"], "winckelmans", sigma)
-    best, rv = cpu_time(lambda: o.P3D_M2M_vel(p, p["coord"], "winckelmans", sigma))
     import oracle
+    cpu = None
+    if not args.no_cpu:
+        best, rv = cpu_time(lambda: o.P3D_M2M_vel(p, p["coord"], "winckelmans", sigma))
+        cpu = {"value": pairs / best, "unit": "interactions/s",
+               "cores": int(o.L.orc_num_threads()), "kind": "port",
+               "sample": f"full {n} x {n} cvtx_P3D_M2M_vel, best of 2, {best:.2f} s",
+               "rel_l2_gpu_vs_cpu": oracle.rel_l2(V, rv)}
     algo_bytes = n * 24 + n * 16 + n * 12
     line = base_line(args, pairs / (ms_step * 1e-3), "interactions/s", ms_step,
                      {"workload": "uniform cube [-1,1]^3, P3D vel, Winckelmans sigma=0.05",
@@ -271,10 +277,7 @@ def bench_cube(args, cv, D, W, o, stream):
     line.update({"roofline": rf, "end_to_end_c_api_ms": e2e_ms,
                  "rel_err_vs_fp64": {"samples": n, "vel": err_stats(V, v64),
                                      "c_api_vel": err_stats(Ve2e, v64)},
-                 "cpu_baseline": {"value": pairs / best, "unit": "interactions/s",
-                                  "cores": int(o.L.orc_num_threads()), "kind": "port",
-                                  "sample": f"full {n} x {n} cvtx_P3D_M2M_vel, best of 2, {best:.2f} s",
-                                  "rel_l2_gpu_vs_cpu": oracle.rel_l2(V, rv)}})
+                 "cpu_baseline": cpu})
     return line
 
 
@@ -307,8 +310,17 @@ def bench_lattice(args, cv, D, W, o, stream):
     v64, d64 = o.f64_F3D(f, t, idx=idx)
     tg = np.ascontiguousarray(t[idx])
     m = min(args.cpu_samples, len(idx))
-    best, (rv, rd) = cpu_time(lambda: (o.F3D_M2M_vel(f, tg["coord"][:m]), o.F3D_M2M_dvort(f, tg[:m])))
     import oracle
+    cpu = None
+    if not args.no_cpu:
+        best, (rv, rd) = cpu_time(lambda: (o.F3D_M2M_vel(f, tg["coord"][:m]),
+                                           o.F3D_M2M_dvort(f, tg[:m])))
+        cpu = {"value": m * float(nf) / best, "unit": "interactions/s",
+               "cores": int(o.L.orc_num_threads()), "kind": "port",
+               "sample": f"{m} sampled targets x {nf} filaments: "
+                         f"cvtx_F3D_M2M_vel then cvtx_F3D_M2M_dvort, best of 2",
+               "rel_l2_gpu_vs_cpu": {"vel": oracle.rel_l2(V[idx[:m]], rv),
+                                     "dvort": oracle.rel_l2(Dv[idx[:m]], rd)}}
     main = res["vel+dvort"]
     line = base_line(args, main["interactions_per_s"], "interactions/s", main["ms_per_step"],
                      {"workload": "128x128 horseshoe sheet of straight filaments on z=0 over "
@@ -317,12 +329,7 @@ def bench_lattice(args, cv, D, W, o, stream):
     line.update({"roofline": main["roofline"], "separate": {k: v for k, v in res.items()},
                  "rel_err_vs_fp64": {"samples": int(len(idx)), "vel": err_stats(V[idx], v64),
                                      "dvort": err_stats(Dv[idx], d64)},
-                 "cpu_baseline": {"value": m * float(nf) / best, "unit": "interactions/s",
-                                  "cores": int(o.L.orc_num_threads()), "kind": "port",
-                                  "sample": f"{m} sampled targets x {nf} filaments: "
-                                            f"cvtx_F3D_M2M_vel then cvtx_F3D_M2M_dvort, best of 2",
-                                  "rel_l2_gpu_vs_cpu": {"vel": oracle.rel_l2(V[idx[:m]], rv),
-                                                        "dvort": oracle.rel_l2(Dv[idx[:m]], rd)}}})
+                 "cpu_baseline": cpu})
     return line
 
 
@@ -388,8 +395,17 @@ def bench_clouds(args, cv, D, W, o, stream):
     cpu_pairs = float(sum(len(cl) ** 2 for cl, _ in subset))
     gv = np.concatenate([e[0] for e in errs_v]); rv = np.concatenate([e[1] for e in errs_v])
     gd = np.concatenate([e[0] for e in errs_d]); rd = np.concatenate([e[1] for e in errs_d])
-    best_cpu, _ = cpu_time(lambda: [(o.P3D_M2M_vel(cl, cl["coord"], "winckelmans", s_),
-                                     o.P3D_M2M_dvort(cl, cl, "winckelmans", s_)) for cl, s_ in subset], 1)
+    cpu = None
+    if not args.no_cpu:
+        best_cpu, _ = cpu_time(lambda: [(o.P3D_M2M_vel(cl, cl["coord"], "winckelmans", s_),
+                                         o.P3D_M2M_dvort(cl, cl, "winckelmans", s_))
+                                        for cl, s_ in subset], 1)
+        cpu = {"value": cpu_pairs / best_cpu, "unit": "interactions/s",
+               "cores": int(o.L.orc_num_threads()), "kind": "port",
+               "sample": "one evaluation (vel then dvort) of 8 clouds spanning "
+                         f"sizes {min(len(c) for c, _ in subset)}.."
+                         f"{max(len(c) for c, _ in subset)}, {cpu_pairs:.3g} "
+                         f"pairs, {best_cpu:.2f} s"}
     F = FLOPS_PER_PAIR[("p3d", "vel+dvort", "winckelmans")]
     line = base_line(args, pairs / (ms_step * 1e-3), "interactions/s", ms_step,
                      {"workload": "512 clouds, sizes log-uniform in [1024, 65536], unit cubes, "
@@ -412,12 +428,7 @@ def bench_clouds(args, cv, D, W, o, stream):
                  "rk4_graph": rk,
                  "rel_err_vs_fp64": {"samples": int(len(gv)), "clouds": [int(c) for c in pick],
                                      "vel": err_stats(gv, rv), "dvort": err_stats(gd, rd)},
-                 "cpu_baseline": {"value": cpu_pairs / best_cpu, "unit": "interactions/s",
-                                  "cores": int(o.L.orc_num_threads()), "kind": "port",
-                                  "sample": "one evaluation (vel then dvort) of 8 clouds spanning "
-                                            f"sizes {min(len(c) for c, _ in subset)}.."
-                                            f"{max(len(c) for c, _ in subset)}, {cpu_pairs:.3g} "
-                                            f"pairs, {best_cpu:.2f} s"}})
+                 "cpu_baseline": cpu})
     b.destroy()
     return line
 
@@ -474,19 +485,22 @@ def bench_sheet(args, cv, D, W, o, stream):
     vz = cv.P2D_M2M_visc_dvort(p[:4096], p[:4096], "planetary", sigma, nu)
     m = min(args.cpu_samples, n)
     tg = np.ascontiguousarray(p[idx[:m]])
-    best, rv = cpu_time(lambda: o.P2D_M2M_vel(p, tg["coord"], "planetary", sigma))
     import oracle
+    cpu = None
+    if not args.no_cpu:
+        best, rv = cpu_time(lambda: o.P2D_M2M_vel(p, tg["coord"], "planetary", sigma))
+        cpu = {"value": m * float(n) / best, "unit": "interactions/s",
+               "cores": int(o.L.orc_num_threads()), "kind": "port",
+               "sample": f"{m} sampled targets x {n} sources cvtx_P2D_M2M_vel, "
+                         f"best of 2, {best:.2f} s",
+               "rel_l2_gpu_vs_cpu": oracle.rel_l2(V[idx[:m]], rv)}
     F = FLOPS_PER_PAIR[("p2d", "vel", "planetary")]
     line = base_line(args, pairs / (ms_step * 1e-3), "interactions/s", ms_step,
                      {"workload": "2D vortex sheet y=0.01 sin(2 pi x), P2D vel, planetary "
                                   "sigma=4/sqrt(N)", "n_particles": n, "sigma": sigma})
     line.update({"roofline": roofline(pairs, F, ms_k), "variants": variants,
                  "rel_err_vs_fp64": err, "visc_exactly_zero": bool(np.all(vz == 0)),
-                 "cpu_baseline": {"value": m * float(n) / best, "unit": "interactions/s",
-                                  "cores": int(o.L.orc_num_threads()), "kind": "port",
-                                  "sample": f"{m} sampled targets x {n} sources cvtx_P2D_M2M_vel, "
-                                            f"best of 2, {best:.2f} s",
-                                  "rel_l2_gpu_vs_cpu": oracle.rel_l2(V[idx[:m]], rv)}})
+                 "cpu_baseline": cpu})
     return line
 
 
