@@ -70,19 +70,26 @@ LaunchCfg choose_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt,
   // Spatially ordered targets (Gaussian P3D, compensated sums).  One target
   // per lane: a 64-target wave of Morton-ordered targets is more compact
   // than a 128-target one, so fewer of its source rounds take the FP64
-  // near-field branch.  And enough source partitions for about 64
-  // workgroups per CU: the near-field work is concentrated in the workgroups
-  // whose targets sit in dense regions, and partitioning spreads each of
-  // them over several CUs instead of leaving them as the tail.  Measured on
-  // the ring (vel+dvort, ms; default before -> this shape):
-  //   2^14: 0.29 -> 0.26,  2^16: 3.73 -> 2.86,  2^18: 58.0 -> 43.1,
-  //   2^20: 710.6 -> 681.0,  2^22: 13742 -> 10581.
+  // near-field branch.  Source partitions: the near-field work is
+  // concentrated in the workgroups whose targets sit in dense regions, and
+  // partitioning spreads each of them over several workgroups instead of
+  // leaving them as the launch's tail.  Enough partitions for about 64
+  // workgroups per CU, and at least one per 65536 sources, but no more than
+  // about 256 workgroups per CU.  Measured on the ring (vel+dvort, ms; one
+  // partition -> this rule, on different boxes):
+  //   2^14: 0.29 -> 0.26 (64 partitions),  2^16: 3.73 -> 2.85 (64),
+  //   2^18: 58.0 -> 42.4 (16; 32 and 64 the same),
+  //   2^20: 710.6 -> 665.5 (16; 4 partitions: 673.6, 32: 665.3),
+  //   2^22: 13742 -> 10564 (4; 1 partition on that box: 10703).
+  // vel alone and vel+dvort+visc at 2^20 also gain 1.2-1.4 % from 4 to 16.
   // The plain FP32 kernel has no FP64 branch and keeps the general shape
   // (two targets per lane: 678 ms against 861 ms with one at 2^20).
   if (kind == KIND_P3D && acc == ACC_COMP && p3d_wants_order(reg, n_src, n_tgt)) {
     const long long blocks = ((long long)n_tgt + 255) / 256;
-    const long long want = 64LL * cus;
-    const long long k = blocks >= want ? 1 : (want + blocks - 1) / blocks;
+    const long long few = (64LL * cus + blocks - 1) / blocks;
+    const long long per = ((long long)n_src + 65535) / 65536;
+    const long long many = (256LL * cus + blocks - 1) / blocks;
+    const long long k = std::min(std::max(few, per), many);
     return launch_shape(1, (int)std::min<long long>(k, 1 << 20), n_src);
   }
   LaunchCfg c = default_cfg(n_src, n_tgt, n_cu);
@@ -140,7 +147,7 @@ std::vector<LaunchCfg> candidates(int n_src, int n_tgt, int n_cu) {
     if (T > 1 && blocks < 8) continue;
     for (int k = 1; k <= 256; k *= 2) {
       if (k > 1 && k > tiles) break;
-      if (blocks * k > 64L * (n_cu > 0 ? n_cu : 256)) break;
+      if (blocks * k > 256L * (n_cu > 0 ? n_cu : 256)) break;
       out.push_back(launch_shape(T, k, n_src));
     }
   }

@@ -88,6 +88,19 @@ def p3d_eval(prob, stream=None, shape=None):
         raise CvtxError(f"p3d_eval: {last_error()}")
 
 
+def autotune_p3d(prob, reps=2):
+    """Time the candidate launch shapes of a device problem and record the
+    fastest for its size bucket (cvtx_hip_autotune_p3d).  Returns
+    (targets_per_lane, n_split, ms)."""
+    L = lib()
+    T, k, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_float(0.0)
+    st = L.cvtx_hip_autotune_p3d(ctypes.byref(prob), reps, ctypes.byref(T), ctypes.byref(k),
+                                 ctypes.byref(ms))
+    if st != 0:
+        raise CvtxError(f"autotune_p3d: {last_error()}")
+    return T.value, k.value, ms.value
+
+
 def vort_eval(prob, stream=None, shape=None):
     L = lib()
     if shape is None:

@@ -169,6 +169,38 @@ def test_ordered_partitioned_gaussian_matches_oracle(orc, what):
         assert oracle.rel_l2(a, b) <= 1e-6
 
 
+def test_autotuner_records_a_shape_that_agrees(tmp_path):
+    # cvtx_hip_autotune_p3d times the candidate launch shapes of a device
+    # problem and caches the fastest for its size bucket; later evaluations
+    # in the bucket use it.  A shape changes how the source sums are split
+    # (tiles per partition, FP64 partials), so results agree with the default
+    # shape to rounding.  The cache round-trips through save / load.
+    case("autotuner: tune, evaluate, save / load / clear")
+    L = cv.lib()
+    L.cvtx_hip_tune_clear()
+    p = W.random_p3d(20000, 95, box=1.0, vort=1.0)
+    dev = D.P3DDevice(p)
+    outs = dev.outputs(True, True, False)
+    prob = dev.problem(outs, "winckelmans", 0.05)
+    D.p3d_eval(prob)
+    v0, d0 = outs[0].download(), outs[1].download()
+    try:
+        T, k, ms = D.autotune_p3d(prob, 1)
+        print(f"  winner: {T} targets per lane, {k} partitions, {ms:.3f} ms", flush=True)
+        assert T in (1, 2, 4) and k >= 1 and ms > 0
+        D.p3d_eval(prob)
+        v1, d1 = outs[0].download(), outs[1].download()
+        assert oracle.rel_l2(v1, v0) <= 1e-6 and oracle.rel_l2(d1, d0) <= 1e-6
+        path = str(tmp_path / "shapes.txt")
+        assert L.cvtx_hip_tune_save(path.encode()) == 0
+        fields = open(path).read().split()
+        assert len(fields) == 9 and int(fields[6]) == T and int(fields[7]) == k
+        L.cvtx_hip_tune_clear()
+        assert L.cvtx_hip_tune_load(path.encode()) == 1
+    finally:
+        L.cvtx_hip_tune_clear()
+
+
 def test_device_api_matches_host_api():
     case("device API == host API")
     p = W.random_p3d(2000, 71, box=1.0, vort=1.0)
