@@ -529,11 +529,13 @@ def bench_relax(args, cv, D, W, o, stream):
     D.relax_eval(dpv, da3, n, fdt, "gaussian", sigma, stream=stream)
     D.synchronize()
     A_gpu = da3.download()[:, :3]
+    # targets per lane and source partitions around the default shape
+    # (one target per lane, 64 partitions)
     sweep = {}
-    for T in (1, 2, 4):
-        _, kT = tm.run(lambda: D.vort_eval(prob, stream, shape=(T, 1)), 2, 1)
-        sweep[f"T={T}"] = kT
-    log(f"  sorted vort by targets per lane: {sweep}")
+    for T, k in ((1, 1), (1, 16), (1, 64), (2, 64), (4, 64)):
+        _, kT = tm.run(lambda: D.vort_eval(prob, stream, shape=(T, k)), 2, 1)
+        sweep[f"T={T},partitions={k}"] = kT
+    log(f"  sorted vort by launch shape: {sweep}")
     # brute force: the same kernel without the spatial sort (every tile evaluated)
     L = cv.lib()
     prev = L.cvtx_hip_set_target_order(0)
@@ -576,7 +578,7 @@ def bench_relax(args, cv, D, W, o, stream):
     line.update({"roofline": dict(roofline(pairs, F, k_b), note="brute-force kernel (no sort): "
                                   "all 2^40 pairs evaluated"),
                  "vort_sorted_ms": k_vort, "vort_brute_force_ms": k_b,
-                 "vort_sorted_ms_by_targets_per_lane": sweep,
+                 "vort_sorted_ms_by_shape": sweep,
                  "speedup_of_tile_skipping": k_b / k_vort,
                  "rel_l2_sorted_vs_brute_force": same,
                  "rel_err_vs_fp64": err, "cpu_baseline": cpu})

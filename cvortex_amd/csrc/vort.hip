@@ -22,6 +22,7 @@
 // result.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "kernels.hpp"
@@ -375,12 +376,28 @@ double vort_scale(int reg, float sigma) {
 }
 
 LaunchCfg vort_default_cfg(int n_src, int n_tgt, int n_cu, bool sorted) {
-  // With tile skipping a smaller target block has a smaller bounding box,
-  // so it skips more source tiles.
-  LaunchCfg c = default_cfg(n_src, n_tgt, n_cu);
-  if (sorted && c.T > VORT_SORTED_T) {
-    c = default_cfg(n_src, n_tgt, n_cu);
-    c.T = VORT_SORTED_T;
+  if (!sorted) return default_cfg(n_src, n_tgt, n_cu);
+  // Sorted sources with tile skipping: one target per lane (a smaller target
+  // block has a smaller bounding box, so it skips more source tiles) and 64
+  // source partitions.  The work left after skipping is very uneven --
+  // target blocks in dense regions keep most of their tiles -- and small
+  // partitions spread it over many workgroups.  Measured on the ring (ms,
+  // no partitions -> 64):
+  //   2^14: 0.14 -> 0.11,  2^16: 1.25 -> 0.40,  2^18: 6.57 -> 2.49,
+  //   2^20: 37.1 -> 22.6 (32 partitions: 23.2, 128: 23.1, 256: 25.1).
+  // The FP64 partial sums (24 B per target and partition) are held to 2^26
+  // target-partitions, 1.5 GiB.
+  LaunchCfg c;
+  c.T = VORT_SORTED_T;
+  const long long tiles = ((long long)n_src + 255) / 256;
+  long long k = std::min<long long>(64, tiles);
+  k = std::min<long long>(k, std::max<long long>(1, (1LL << 26) / std::max(n_tgt, 1)));
+  if (k <= 1) return c;
+  c.split_len = (int)((tiles + k - 1) / k * 256);
+  c.n_split = (int)(((long long)n_src + c.split_len - 1) / c.split_len);
+  if (c.n_split <= 1) {
+    c.n_split = 1;
+    c.split_len = 0;
   }
   return c;
 }
