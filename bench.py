@@ -376,6 +376,10 @@ def bench_clouds(args, cv, D, W, o, stream):
         s, k = tm.run(lambda: b.rk4_step(1, stream), args.rk4_steps, 0)
         rk = {"steps": args.rk4_steps, "ms_per_step": s, "gpu_ms_per_step": k,
               "interactions_per_s": 4 * pairs / (s * 1e-3), "dt": 1e-3}
+        # the state after warm-up + timed steps: finite, and how far it moved
+        xs, al = dpv.download(), da.download()
+        rk["state_finite"] = bool(np.all(np.isfinite(xs)) and np.all(np.isfinite(al)))
+        rk["max_displacement"] = float(np.abs(xs[:, :3] - pv[:, :3]).max())
         log(f"  RK4 graph step {s:.1f} ms")
     # accuracy: 256 targets in each of 8 clouds spanning the size range
     order = np.argsort(sizes)
