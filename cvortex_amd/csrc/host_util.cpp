@@ -87,16 +87,34 @@ void build_gaussian_table(float* out) {
 }
 
 LaunchCfg default_cfg(int n_src, int n_tgt, int n_cu) {
+  const long long cus = n_cu > 0 ? n_cu : 256;
+  const long long want = 4 * cus;  // workgroups to fill the chip
+  auto blocks = [&](int T) { return ((long long)n_tgt + 256LL * T - 1) / (256LL * T); };
+  int T = 1;
+  if (blocks(4) >= 2 * want) T = 4;
+  else if (blocks(2) >= want / 2) T = 2;
+  return partitioned_cfg(T, n_src, n_tgt, n_cu);
+}
+
+LaunchCfg partitioned_cfg(int T, int n_src, int n_tgt, int n_cu) {
   LaunchCfg c;
   c.wg = 256;
-  const long long want = 4LL * (n_cu > 0 ? n_cu : 256);  // workgroups to fill the chip
-  auto blocks = [&](int T) { return ((long long)n_tgt + 256LL * T - 1) / (256LL * T); };
-  if (blocks(4) >= 2 * want) c.T = 4;
-  else if (blocks(2) >= want / 2) c.T = 2;
-  else c.T = 1;
-  const long long nb = blocks(c.T);
+  c.T = T;
+  const long long cus = n_cu > 0 ? n_cu : 256;
+  const long long nb = ((long long)n_tgt + 256LL * T - 1) / (256LL * T);
   const long long tiles = ((long long)n_src + 255) / 256;
-  long long k = nb >= want ? 1 : (want + nb - 1) / nb;
+  // Source partitions, for two reasons:
+  //   * to fill the chip (4 workgroups per CU) when there are few target
+  //     blocks, down to one tile per partition;
+  //   * for an even finish: with uniform work per pair, a launch of ~16
+  //     workgroups per CU still ends unevenly across the CUs, and ~64 per CU
+  //     does better.  Measured: 16384 filaments x 2^20 targets, vel
+  //     16.54 -> 16.12 ms (T=1, 4 partitions), vel+dvort 24.43 -> 23.61 ms
+  //     (T=2, 4); 2^22 P2D sheet 3577 -> 3501 ms (T=4, 4).  These
+  //     partitions keep at least 16 tiles each.
+  const long long fill = nb >= 4 * cus ? 1 : (4 * cus + nb - 1) / nb;
+  const long long even = nb >= 64 * cus ? 1 : (64 * cus + nb - 1) / nb;
+  long long k = std::max(fill, std::min(even, std::max(1LL, tiles / 16)));
   k = std::min(k, std::max(1LL, tiles));
   if (k <= 1) {
     c.n_split = 1;
@@ -105,7 +123,7 @@ LaunchCfg default_cfg(int n_src, int n_tgt, int n_cu) {
   }
   const long long tiles_per = (tiles + k - 1) / k;
   c.split_len = (int)(tiles_per * 256);
-  c.n_split = (int)((n_src + c.split_len - 1) / c.split_len);
+  c.n_split = (int)(((long long)n_src + c.split_len - 1) / c.split_len);
   if (c.n_split <= 1) {
     c.n_split = 1;
     c.split_len = 0;

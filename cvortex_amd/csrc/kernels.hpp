@@ -121,6 +121,8 @@ size_t f3d_workspace_bytes(const F3DJob& j, const LaunchCfg& c);
 
 // Default launch shape for a problem size (the tuner may override).
 LaunchCfg default_cfg(int n_src, int n_tgt, int n_cu);
+// The default source partitions for T targets per lane.
+LaunchCfg partitioned_cfg(int T, int n_src, int n_tgt, int n_cu);
 
 // Enqueue the kernels on `s`.  Return hipSuccess or the launch error.
 // Valid combinations: P3D visc only with Winckelmans or Gaussian.

@@ -97,12 +97,11 @@ LaunchCfg choose_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt,
   // Filaments with one output (vel or dvort alone): one target per lane fits
   // in 48-50 VGPRs, eight waves per SIMD against five or six with two, and
   // the extra waves pair more instructions for dual issue than the second
-  // target's reuse saves (16384 filaments x 2^20 targets: vel 16.9 ms against
-  // 17.6, dvort 23.1 against 24.4; the fused kernel keeps two, 24.4 against
-  // 25.1).
-  if (kind == KIND_F3D && (mask == OUT_VEL || mask == OUT_DVORT) && c.T == 2 &&
-      c.n_split == 1 && many_blocks)
-    c.T = 1;
+  // target's reuse saves (16384 filaments x 2^20 targets, no partitions: vel
+  // 16.9 ms against 17.6, dvort 23.1 against 24.4; with 4 partitions vel
+  // 16.1 against 17.6; the fused kernel keeps two, 24.4 against 25.1).
+  if (kind == KIND_F3D && (mask == OUT_VEL || mask == OUT_DVORT) && c.T == 2 && many_blocks)
+    c = partitioned_cfg(1, n_src, n_tgt, n_cu);
   return c;
 }
 
