@@ -103,6 +103,7 @@ LaunchCfg partitioned_cfg(int T, int n_src, int n_tgt, int n_cu) {
   const long long cus = n_cu > 0 ? n_cu : 256;
   const long long nb = ((long long)n_tgt + 256LL * T - 1) / (256LL * T);
   const long long tiles = ((long long)n_src + 255) / 256;
+  if (nb <= 0 || tiles <= 0) return c;  // nothing to split (no targets or sources)
   // Source partitions, for two reasons:
   //   * to fill the chip (4 workgroups per CU) when there are few target
   //     blocks, down to one tile per partition;

@@ -201,6 +201,33 @@ def test_autotuner_records_a_shape_that_agrees(tmp_path):
         L.cvtx_hip_tune_clear()
 
 
+def test_device_api_zero_targets():
+    # device problems with sources but no targets are no-ops for every kind
+    # (the launch-shape choice sees zero target blocks), and the library
+    # keeps working afterwards
+    case("device API with zero targets")
+    p = W.random_p3d(1000, 97, box=1.0, vort=1.0)
+    dev = D.P3DDevice(p)
+    v, dv, _ = dev.outputs(True, True, False)
+    for k in ("winckelmans", "gaussian"):
+        D.p3d_eval(D.p3d_problem(dev.pv, dev.a, 1000, dev.pv, dev.a, 0, v, dv, None, k, 0.1))
+    q = D.DeviceArray.from_host(D.p2d_soa(W.random_p2d(1000, 98, box=1.0, vort=1.0)))
+    v2 = D.DeviceArray((1000, 2))
+    D.p2d_eval(D.P2DProblem(q.addr, 1000, q.addr, 0, v2.addr, None, D.REG_ID["winckelmans"], 0.1,
+                            0.0, cv.ACC_COMP))
+    fa, fb = D.f3d_soa(W.random_f3d(700, 99, box=1.0, strength=1.0))
+    dfa, dfb = D.DeviceArray.from_host(fa), D.DeviceArray.from_host(fb)
+    D.f3d_eval(D.F3DProblem(dfa.addr, dfb.addr, 700, dev.pv.addr, dev.a.addr, 0, v.addr, dv.addr,
+                            cv.ACC_COMP))
+    om = D.DeviceArray((1000, 3))
+    D.vort_eval(D.VortProblem(dev.pv.addr, dev.a.addr, 1000, dev.pv.addr, 0, om.addr,
+                              D.REG_ID["gaussian"], 0.1, cv.ACC_COMP))
+    D.synchronize()
+    D.p3d_eval(dev.problem((v, dv, None), "winckelmans", 0.1))
+    hv, hd, _ = cv.P3D_M2M_fused(p, p, "winckelmans", 0.1, 0.0, vel=True, dvort=True)
+    assert np.array_equal(v.download(), hv) and np.array_equal(dv.download(), hd)
+
+
 def test_device_api_matches_host_api():
     case("device API == host API")
     p = W.random_p3d(2000, 71, box=1.0, vort=1.0)
