@@ -251,9 +251,10 @@ static inline bsv_V3f p3d_visc_pair(const cvtx_P3D* self, const cvtx_P3D* ind,
 	bsv_V3f d = v3_sub(self->coord, ind->coord);
 	float r = v3_norm(d);
 	float rho = fabsf(r / sigma);
-	float pre = 2.f * nu / powf(sigma, 2.f);
-	bsv_V3f t1 = v3_scale(self->vorticity, ind->volume);
-	bsv_V3f t2 = v3_scale(ind->vorticity, (ORC_MUT(5) ? 1.f : -1.f) * self->volume);
+	float pre = (ORC_MUT(20) ? 1.f : 2.f) * nu / powf(sigma, 2.f);
+	bsv_V3f t1 = v3_scale(self->vorticity, ORC_MUT(21) ? self->volume : ind->volume);
+	bsv_V3f t2 = v3_scale(ind->vorticity, (ORC_MUT(5) ? 1.f : -1.f) *
+	                                      (ORC_MUT(21) ? ind->volume : self->volume));
 	bsv_V3f t = v3_add(t1, t2);
 	float eta = f->eta3(rho);
 	return v3_scale(v3_scale(t, eta), pre);
@@ -835,9 +836,9 @@ static inline float p2d_visc_pair(const cvtx_P2D* self, const cvtx_P2D* ind,
 	bsv_V2f d = v2_sub(self->coord, ind->coord);
 	float r = v2_norm(d);
 	float rho = fabsf(r / sigma);
-	float pre = 2.f * nu / powf(sigma, 2.f);
-	float a = self->vorticity * ind->area;
-	float b = -ind->vorticity * self->area;
+	float pre = (ORC_MUT(22) ? 1.f : 2.f) * nu / powf(sigma, 2.f);
+	float a = self->vorticity * (ORC_MUT(23) ? self->area : ind->area);
+	float b = -ind->vorticity * (ORC_MUT(23) ? ind->area : self->area);
 	float t = a + b;
 	float eta = f->eta2(rho);
 	return t * eta * pre;

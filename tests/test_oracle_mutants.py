@@ -37,6 +37,10 @@ EXPECTED = {
     17: "redistribution_moments",  # m4p inner coefficient 2.5 -> 2.0
     18: "redistribution_moments",  # removed strength not redistributed
     19: "f3d_dvort_second_statement",  # F3D dvort isotropic coefficient 3 -> 2
+    20: "pse_laplacian",        # 3D PSE prefactor 2 nu / sigma^2 -> nu / sigma^2
+    21: "pse_laplacian",        # 3D PSE volumes exchanged: a_s V_s - a_t V_t
+    22: "pse_laplacian",        # 2D PSE prefactor 2 nu / sigma^2 -> nu / sigma^2
+    23: "pse_laplacian",        # 2D PSE areas exchanged: a_s A_s - a_t A_t
 }
 
 
