@@ -167,6 +167,7 @@ def bench_ring(args, cv, D, W, o, stream):
         variants["plain_fp32"] = {"interactions_per_s": pairs / (s2 * 1e-3), "kernel_ms": k2,
                                   "frac_of_peak": roofline(pairs, F, k2)["frac"]}
         plain = (o2[0].download(), o2[1].download())
+    visc = None
     if "visc" in args.variants:
         o3 = dev.outputs(True, True, True)
         s3, k3 = tm.run(runner(o3, cv.ACC_COMP, True), max(1, args.steps - 1), 1)
@@ -174,6 +175,7 @@ def bench_ring(args, cv, D, W, o, stream):
         variants["vel_dvort_visc"] = {"interactions_per_s": pairs / (s3 * 1e-3), "kernel_ms": k3,
                                       "tflops": roofline(pairs, F3, k3)["achieved"],
                                       "frac_of_peak": roofline(pairs, F3, k3)["frac"]}
+        visc = o3[2].download()
     if "c_api" in args.variants:
         # the host-array boundary: gather from the caller's pointer arrays,
         # copies over PCIe both ways, kernels, all inside the timed call
@@ -198,9 +200,12 @@ def bench_ring(args, cv, D, W, o, stream):
         rng = np.random.default_rng(0x5EED)
         idx = np.sort(rng.choice(n, size=min(n_err(args, 4096), n), replace=False)).astype(np.int32)
         log(f"FP64 oracle on {len(idx)} sampled targets...")
-        v64, d64, _ = o.f64_P3D(p, p, "gaussian", sigma, idx=idx, gauss_mode=1)
+        v64, d64, e64 = o.f64_P3D(p, p, "gaussian", sigma, nu, idx=idx, gauss_mode=1,
+                                  visc=visc is not None)
         err = {"samples": int(len(idx)), "vel": err_stats(vel[idx], v64),
                "dvort": err_stats(dvort[idx], d64)}
+        if visc is not None:
+            err["visc"] = err_stats(visc[idx], e64)
         if plain is not None:
             err["plain_fp32"] = {"vel": err_stats(plain[0][idx], v64),
                                  "dvort": err_stats(plain[1][idx], d64)}

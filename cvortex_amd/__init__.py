@@ -26,6 +26,11 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcvortex.
 ACC_PLAIN = 0
 ACC_COMP = 1
 OUT_VEL, OUT_DVORT, OUT_VISC = 1, 2, 4
+# which leg the public void functions use (cvtx_hip_set_dispatch)
+DISPATCH_AUTO, DISPATCH_DEVICE, DISPATCH_HOST = 0, 1, 2
+LEG_DEVICE, LEG_HOST = 0, 1
+# single-source / single-target calls from this many elements go to the device
+SMALL_DEVICE_MIN = 65536
 
 _lib = None
 
@@ -114,6 +119,10 @@ _SIGNATURES = {
     "cvtx_hip_get_accumulation": (_I, []),
     "cvtx_hip_set_target_order": (_I, [_I]),
     "cvtx_hip_stream": (_VP, []),
+    "cvtx_hip_set_dispatch": (_I, [_I]),
+    "cvtx_hip_get_dispatch": (_I, []),
+    "cvtx_hip_last_leg": (_I, []),
+    "cvtx_hip_host_threads": (_I, []),
     "cvtx_hip_synchronize": (_I, []),
     # device-resident
     "cvtx_hip_p3d_eval": (_I, [_VP, _VP]),
@@ -219,6 +228,28 @@ def set_accumulation(mode):
 
 def get_accumulation():
     return lib().cvtx_hip_get_accumulation()
+
+
+def set_dispatch(mode):
+    """Leg of the public void functions (DISPATCH_AUTO, _DEVICE, _HOST);
+    returns the previous mode."""
+    prev = lib().cvtx_hip_set_dispatch(mode)
+    if prev < 0:
+        raise CvtxError(f"invalid dispatch mode {mode}")
+    return prev
+
+
+def get_dispatch():
+    return lib().cvtx_hip_get_dispatch()
+
+
+def last_leg():
+    """LEG_DEVICE or LEG_HOST for the last public void call (-1: none or failed)."""
+    return lib().cvtx_hip_last_leg()
+
+
+def host_threads():
+    return lib().cvtx_hip_host_threads()
 
 
 def VortFunc_singular():

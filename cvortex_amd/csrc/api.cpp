@@ -5,7 +5,8 @@
 // Host-array calls follow one path: gather the caller's pointer arrays into
 // pinned structure-of-arrays staging (parallel over host threads), copy to
 // the device on the library stream, run the kernels, copy the results back
-// and synchronise.  The library never evaluates a pair on the CPU.
+// and synchronise.  The seam never evaluates a pair on the CPU: the public
+// void functions (dispatch.cpp) do that when the seam declines.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -165,6 +166,7 @@ int p3d_host(const cvtx_P3D** src, int ns, const bsv_V3f* mes, const cvtx_P3D** 
   if (mask == 0 || nt == 0) return 0;
   Device* d = active_device(rt);
   if (!d) return -1;
+  StreamUse use(*d, d->stream);
 
   const size_t sb = sizeof(float4) * (size_t)(ns > 0 ? ns : 1);
   const size_t tb = sizeof(float4) * (size_t)nt;
@@ -244,6 +246,7 @@ int f3d_host(const cvtx_F3D** src, int nf, const bsv_V3f* mes, const cvtx_P3D** 
   if (mask == 0 || nt == 0) return 0;
   Device* d = active_device(rt);
   if (!d) return -1;
+  StreamUse use(*d, d->stream);
   const size_t sb = sizeof(float4) * (size_t)(nf > 0 ? nf : 1);
   const size_t tb = sizeof(float4) * (size_t)nt;
   const size_t ob = sizeof(float) * 3 * (size_t)nt;
@@ -328,6 +331,7 @@ int p2d_host(const cvtx_P2D** src, int ns, const bsv_V2f* mes, const cvtx_P2D** 
   if (mask == 0 || nt == 0) return 0;
   Device* d = active_device(rt);
   if (!d) return -1;
+  StreamUse use(*d, d->stream);
   const size_t sb = sizeof(float4) * (size_t)(ns > 0 ? ns : 1);
   const size_t tb = sizeof(float4) * (size_t)nt;
   float4* h_s = (float4*)host_buffer(rt, *d, SLOT_SRC0, sb);
@@ -516,53 +520,6 @@ int cvtx_hip_P2D_M2M_fused(const cvtx_P2D** src, int ns, const cvtx_P2D** ind, i
   return p2d_host(src, ns, nullptr, ind, ni, (float*)vel, visc, k, sigma, nu);
 }
 
-// ============================ public M2M API ============================
-// reference src/P3D.cpp:343-456, src/F3D.cpp:162-202, src/P2D.cpp:141-276
-
-void cvtx_P3D_M2M_vel(const cvtx_P3D** src, const int ns, const bsv_V3f* mes, const int nm,
-                      bsv_V3f* out, const cvtx_VortFunc* k, float sigma) {
-  if (cvtx_hip_P3D_M2M_vel(src, ns, mes, nm, out, k, sigma) != 0)
-    report_failure("cvtx_P3D_M2M_vel", (float*)out, 3L * (nm > 0 ? nm : 0));
-}
-
-void cvtx_P3D_M2M_dvort(const cvtx_P3D** src, const int ns, const cvtx_P3D** ind, const int ni,
-                        bsv_V3f* out, const cvtx_VortFunc* k, float sigma) {
-  if (cvtx_hip_P3D_M2M_dvort(src, ns, ind, ni, out, k, sigma) != 0)
-    report_failure("cvtx_P3D_M2M_dvort", (float*)out, 3L * (ni > 0 ? ni : 0));
-}
-
-void cvtx_P3D_M2M_visc_dvort(const cvtx_P3D** src, const int ns, const cvtx_P3D** ind,
-                             const int ni, bsv_V3f* out, const cvtx_VortFunc* k, float sigma,
-                             float nu) {
-  if (cvtx_hip_P3D_M2M_visc_dvort(src, ns, ind, ni, out, k, sigma, nu) != 0)
-    report_failure("cvtx_P3D_M2M_visc_dvort", (float*)out, 3L * (ni > 0 ? ni : 0));
-}
-
-void cvtx_F3D_M2M_vel(const cvtx_F3D** src, const int nf, const bsv_V3f* mes, const int nm,
-                      bsv_V3f* out) {
-  if (cvtx_hip_F3D_M2M_vel(src, nf, mes, nm, out) != 0)
-    report_failure("cvtx_F3D_M2M_vel", (float*)out, 3L * (nm > 0 ? nm : 0));
-}
-
-void cvtx_F3D_M2M_dvort(const cvtx_F3D** src, const int nf, const cvtx_P3D** ind, const int ni,
-                        bsv_V3f* out) {
-  if (cvtx_hip_F3D_M2M_dvort(src, nf, ind, ni, out) != 0)
-    report_failure("cvtx_F3D_M2M_dvort", (float*)out, 3L * (ni > 0 ? ni : 0));
-}
-
-void cvtx_P2D_M2M_vel(const cvtx_P2D** src, const int ns, const bsv_V2f* mes, const int nm,
-                      bsv_V2f* out, const cvtx_VortFunc* k, float sigma) {
-  if (cvtx_hip_P2D_M2M_vel(src, ns, mes, nm, out, k, sigma) != 0)
-    report_failure("cvtx_P2D_M2M_vel", (float*)out, 2L * (nm > 0 ? nm : 0));
-}
-
-void cvtx_P2D_M2M_visc_dvort(const cvtx_P2D** src, const int ns, const cvtx_P2D** ind,
-                             const int ni, float* out, const cvtx_VortFunc* k, float sigma,
-                             float nu) {
-  if (cvtx_hip_P2D_M2M_visc_dvort(src, ns, ind, ni, out, k, sigma, nu) != 0)
-    report_failure("cvtx_P2D_M2M_visc_dvort", out, (long)(ni > 0 ? ni : 0));
-}
-
 // ============================== controls ==============================
 
 const char* cvtx_hip_last_error(void) { return runtime().last_error.c_str(); }
@@ -612,6 +569,7 @@ static int p3d_dev(const cvtx_hip_p3d_problem* p, const LaunchCfg* forced, void*
   }
   Device* d = active_device(rt);
   if (!d) return -1;
+  StreamUse use(*d, pick_stream(d, stream));
   P3DJob j;
   j.src_pv = (const float4*)p->src_pv;
   j.src_a = (const float4*)p->src_a;
@@ -660,6 +618,7 @@ static int p2d_dev(const cvtx_hip_p2d_problem* p, const LaunchCfg* forced, void*
   }
   Device* d = active_device(rt);
   if (!d) return -1;
+  StreamUse use(*d, pick_stream(d, stream));
   P2DJob j;
   j.src = (const float4*)p->src;
   j.n_src = p->n_src;
@@ -698,6 +657,7 @@ static int f3d_dev(const cvtx_hip_f3d_problem* p, const LaunchCfg* forced, void*
   }
   Device* d = active_device(rt);
   if (!d) return -1;
+  StreamUse use(*d, pick_stream(d, stream));
   F3DJob j;
   j.fa = (const float4*)p->fa;
   j.fb = (const float4*)p->fb;

@@ -6,6 +6,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include <cvortex/libcvtx.h>
 
 #include "kernels.hpp"
@@ -41,6 +43,38 @@ int vort_host(const cvtx_P3D** src, int ns, const bsv_V3f* mes, int nt, float* o
 // *out_dev holds the packed records (7 floats per P3D, 4 per P2D) in a
 // library buffer, valid until the next call.
 int redistribute(Runtime& rt, Device& d, const RedistJob& j, float** out_dev, hipStream_t s);
+// Public-function dispatch (dispatch.cpp).  want_device: whether to offer a
+// call to the seam (the dispatch mode, an enabled device, a known kernel
+// name when vf is given, and the reference's size guard when the mode is
+// automatic); device_done / host_done record the leg; device_declined
+// reports a failed device call that falls back to the host; host_args_ok
+// reports invalid arguments (NaN results) and returns false.
+bool want_device(bool guard_met, const cvtx_VortFunc* vf);
+void device_done();
+void device_declined(const char* fn);
+bool host_args_ok(const char* fn, bool ok, float* out, long n_out);
+void host_done(const char* fn, int status, const std::string& err, float* out, long n_out);
+// One public evaluation: the device when want_device(guard, ...) agrees and
+// the seam succeeds, else the host leg (dispatch.cpp).
+void run_p3d_vel(const char* fn, bool guard, const cvtx_P3D** src, int ns, const bsv_V3f* mes,
+                 int nm, bsv_V3f* out, const cvtx_VortFunc* k, float sigma);
+void run_p3d_dvort(const char* fn, bool guard, const cvtx_P3D** src, int ns,
+                   const cvtx_P3D** ind, int ni, bsv_V3f* out, const cvtx_VortFunc* k,
+                   float sigma);
+void run_p3d_visc(const char* fn, bool guard, const cvtx_P3D** src, int ns,
+                  const cvtx_P3D** ind, int ni, bsv_V3f* out, const cvtx_VortFunc* k,
+                  float sigma, float nu);
+void run_p3d_vort(const char* fn, bool guard, const cvtx_P3D** src, int ns, const bsv_V3f* mes,
+                  int nm, bsv_V3f* out, const cvtx_VortFunc* k, float sigma, bool cutoff);
+void run_f3d_vel(const char* fn, bool guard, const cvtx_F3D** src, int nf, const bsv_V3f* mes,
+                 int nm, bsv_V3f* out);
+void run_f3d_dvort(const char* fn, bool guard, const cvtx_F3D** src, int nf,
+                   const cvtx_P3D** ind, int ni, bsv_V3f* out);
+void run_p2d_vel(const char* fn, bool guard, const cvtx_P2D** src, int ns, const bsv_V2f* mes,
+                 int nm, bsv_V2f* out, const cvtx_VortFunc* k, float sigma);
+void run_p2d_visc(const char* fn, bool guard, const cvtx_P2D** src, int ns,
+                  const cvtx_P2D** ind, int ni, float* out, const cvtx_VortFunc* k, float sigma,
+                  float nu);
 // explicit launch shape: T in {1, 2, 4}, n_split >= 1 source partitions
 // (rounded to whole 256-source tiles); false for an invalid shape
 bool make_launch_shape(int T, int n_split, int n_src, LaunchCfg* c);

@@ -149,7 +149,8 @@ BFn b_pick(int reg, int mask, int acc, int T) {
 using namespace cvtx;
 
 struct cvtx_hip_batch {
-  Device* dev = nullptr;
+  Device* dev = nullptr;  // valid while gen equals the runtime's generation
+  unsigned gen = 0;
   int n_clouds = 0;
   int n_particles = 0;
   int reg = REG_WINCKELMANS;
@@ -180,16 +181,47 @@ struct cvtx_hip_batch {
 
 namespace {
 
-void batch_free(cvtx_hip_batch* b) {
-  if (!b) return;
+// RK4 stage buffers, capture stream and graph (also after a failed init)
+void rk4_release(cvtx_hip_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
-  void* ptrs[] = {b->d_units, b->d_range, b->d_rc, b->d_sc, b->d_counter, b->d_work,
-                  b->ysp, b->ysa, b->Sp, b->Sa, b->kv, b->kd};
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->cap_stream = nullptr;
+  float4** f4[] = {&b->ysp, &b->ysa, &b->Sp, &b->Sa};
+  for (float4** p : f4) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  float** f1[] = {&b->kv, &b->kd};
+  for (float** p : f1) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+}
+
+void batch_free(cvtx_hip_batch* b) {
+  if (!b) return;
+  rk4_release(b);
+  void* ptrs[] = {b->d_units, b->d_range, b->d_rc, b->d_sc, b->d_counter, b->d_work};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete b;
+}
+
+// A batch holds a pointer into the runtime's device list and a graph that
+// names the device's Gaussian table: both end with cvtx_finalise.
+bool batch_live(Runtime& rt, const cvtx_hip_batch* b) {
+  if (!b) {
+    set_error(rt, "batch: null batch");
+    return false;
+  }
+  if (b->gen != rt.generation || !rt.initialised) {
+    set_error(rt, "batch: made before cvtx_finalise; destroy it and make a new one");
+    return false;
+  }
+  return true;
 }
 
 bool build_units(cvtx_hip_batch* b, int T) {
@@ -283,6 +315,7 @@ cvtx_hip_batch* cvtx_hip_batch_create(int n_clouds, const int* offsets, const fl
   if (!d) return nullptr;
   cvtx_hip_batch* b = new cvtx_hip_batch;
   b->dev = d;
+  b->gen = rt.generation;
   b->n_clouds = n_clouds;
   b->n_particles = offsets[n_clouds];
   b->reg = regularisation;
@@ -331,7 +364,8 @@ void cvtx_hip_batch_destroy(cvtx_hip_batch* b) {
 int cvtx_hip_batch_set_shape(cvtx_hip_batch* b, int targets_per_lane, int workgroups_per_cu) {
   Runtime& rt = runtime();
   std::lock_guard<std::recursive_mutex> lk(rt.mu);
-  if (!b || (targets_per_lane != 1 && targets_per_lane != 2 && targets_per_lane != 4) ||
+  if (!batch_live(rt, b)) return -1;
+  if ((targets_per_lane != 1 && targets_per_lane != 2 && targets_per_lane != 4) ||
       workgroups_per_cu < 1 || workgroups_per_cu > 64 || b->exec) {
     set_error(rt, "batch: invalid shape (or RK4 graph already captured)");
     return -1;
@@ -348,7 +382,8 @@ int cvtx_hip_batch_eval(cvtx_hip_batch* b, const float* pv, const float* a, floa
                         float* dvort, float* visc, void* stream) {
   Runtime& rt = runtime();
   std::lock_guard<std::recursive_mutex> lk(rt.mu);
-  if (!b || (b->n_particles > 0 && (!pv || !a || !vel || !dvort)) ||
+  if (!batch_live(rt, b)) return -1;
+  if ((b->n_particles > 0 && (!pv || !a || !vel || !dvort)) ||
       ((b->mask & OUT_VISC) && b->n_particles > 0 && !visc)) {
     set_error(rt, "batch: null buffer");
     return -1;
@@ -363,11 +398,14 @@ int cvtx_hip_batch_eval_per_cloud(cvtx_hip_batch* b, const float* pv, const floa
                                   float* dvort, float* visc, int plain, void* stream) {
   Runtime& rt = runtime();
   std::lock_guard<std::recursive_mutex> lk(rt.mu);
-  if (!b || (b->n_particles > 0 && (!pv || !a || !vel || !dvort))) {
+  if (!batch_live(rt, b)) return -1;
+  if ((b->n_particles > 0 && (!pv || !a || !vel || !dvort)) ||
+      ((b->mask & OUT_VISC) && b->n_particles > 0 && !visc)) {
     set_error(rt, "batch: null buffer");
     return -1;
   }
   hipStream_t s = stream ? (hipStream_t)stream : b->dev->stream;
+  StreamUse use(*b->dev, s);  // the partitioned launches share the device's workspace
   const float4* P = (const float4*)pv;
   const float4* A = (const float4*)a;
   for (int c = 0; c < b->n_clouds; ++c) {
@@ -413,7 +451,8 @@ int cvtx_hip_batch_eval_per_cloud(cvtx_hip_batch* b, const float* pv, const floa
 int cvtx_hip_batch_rk4_init(cvtx_hip_batch* b, float* pv, float* a, float dt) {
   Runtime& rt = runtime();
   std::lock_guard<std::recursive_mutex> lk(rt.mu);
-  if (!b || b->exec || b->mask != (OUT_VEL | OUT_DVORT) || (b->n_particles > 0 && (!pv || !a))) {
+  if (!batch_live(rt, b)) return -1;
+  if (b->exec || b->mask != (OUT_VEL | OUT_DVORT) || (b->n_particles > 0 && (!pv || !a))) {
     set_error(rt, "batch: RK4 needs a vel|dvort batch, buffers, and no earlier init");
     return -1;
   }
@@ -428,6 +467,7 @@ int cvtx_hip_batch_rk4_init(cvtx_hip_batch* b, float* pv, float* a, float dt) {
   ok = ok && hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) == hipSuccess;
   if (!ok) {
     set_error(rt, "batch: RK4 allocation failed");
+    rk4_release(b);
     return -1;
   }
   b->y0p = (float4*)pv;
@@ -435,8 +475,10 @@ int cvtx_hip_batch_rk4_init(cvtx_hip_batch* b, float* pv, float* a, float dt) {
   const int N = b->n_particles;
   const dim3 g((N + 255) / 256 > 0 ? (N + 255) / 256 : 1), blk(256);
   hipStream_t s = b->cap_stream;
-  if (!check_hip(rt, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture"))
+  if (!check_hip(rt, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture")) {
+    rk4_release(b);
     return -1;
+  }
   hipError_t e = hipSuccess;
   auto eval = [&](const float4* p_, const float4* a_) {
     if (e == hipSuccess) e = enqueue_eval(b, (const float*)p_, (const float*)a_, b->kv, b->kd, nullptr, s);
@@ -460,9 +502,13 @@ int cvtx_hip_batch_rk4_init(cvtx_hip_batch* b, float* pv, float* a, float dt) {
     e = hipGetLastError();
   }
   hipError_t e2 = hipStreamEndCapture(s, &b->graph);
-  if (!check_hip(rt, e, "RK4 capture") || !check_hip(rt, e2, "end capture")) return -1;
-  if (!check_hip(rt, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate"))
+  if (!check_hip(rt, e, "RK4 capture") || !check_hip(rt, e2, "end capture") ||
+      !check_hip(rt, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0),
+                 "graph instantiate")) {
+    b->exec = nullptr;
+    rk4_release(b);
     return -1;
+  }
   return 0;
 }
 
@@ -470,7 +516,8 @@ int cvtx_hip_batch_rk4_init(cvtx_hip_batch* b, float* pv, float* a, float dt) {
 int cvtx_hip_batch_rk4_step(cvtx_hip_batch* b, int n_steps, void* stream) {
   Runtime& rt = runtime();
   std::lock_guard<std::recursive_mutex> lk(rt.mu);
-  if (!b || !b->exec || n_steps < 0) {
+  if (!batch_live(rt, b)) return -1;
+  if (!b->exec || n_steps < 0) {
     set_error(rt, "batch: RK4 not initialised");
     return -1;
   }

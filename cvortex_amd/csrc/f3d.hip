@@ -321,6 +321,10 @@ hipError_t launch_f3d(const F3DJob& j, const LaunchCfg& c, hipStream_t s) {
   if (j.n_tgt <= 0) return hipSuccess;
   if (j.n_fil < 0 || c.wg != 256) return hipErrorInvalidValue;
   if ((j.mask & OUT_DVORT) && j.tgt_a == nullptr) return hipErrorInvalidValue;
+  // every output the mask selects must exist, and the inputs it reads
+  if (((j.mask & OUT_VEL) && !j.out_vel) || ((j.mask & OUT_DVORT) && !j.out_dvort) || !j.tgt_pv ||
+      (j.n_fil > 0 && (!j.fa || !j.fb)))
+    return hipErrorInvalidValue;
   F3Fn k = f3_pick(j.mask, j.acc, c.T);
   if (!k) return hipErrorInvalidValue;
   F3DKParams p{};

@@ -34,6 +34,7 @@ int inf_host(const cvtx_F3D** fils, int nf, const bsv_V3f* mes, const bsv_V3f* d
   }
   Device* d = active_device(rt);
   if (!d) return -1;
+  StreamUse use(*d, d->stream);
   const size_t fb = sizeof(float4) * (size_t)nf;
   const size_t mb = sizeof(float4) * (size_t)nm;
   const size_t ob = sizeof(float) * (size_t)nm * (size_t)nf;
@@ -71,15 +72,6 @@ int cvtx_hip_F3D_inf_mtrx(const cvtx_F3D** array_start, int num_filaments,
                           const bsv_V3f* mes_start, const bsv_V3f* dir_start, int num_mes,
                           float* result_matrix) {
   return inf_host(array_start, num_filaments, mes_start, dir_start, num_mes, result_matrix);
-}
-
-void cvtx_F3D_inf_mtrx(const cvtx_F3D** array_start, const int num_filaments,
-                       const bsv_V3f* mes_start, const bsv_V3f* dir_start, const int num_mes,
-                       float* result_matrix) {
-  if (cvtx_hip_F3D_inf_mtrx(array_start, num_filaments, mes_start, dir_start, num_mes,
-                            result_matrix) != 0)
-    report_failure("cvtx_F3D_inf_mtrx", result_matrix,
-                   (long)(num_mes > 0 ? num_mes : 0) * (long)(num_filaments > 0 ? num_filaments : 0));
 }
 
 int cvtx_hip_f3d_inf_eval(const float* fa, const float* fb, int n_fil, const float* mes,

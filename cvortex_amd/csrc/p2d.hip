@@ -286,6 +286,10 @@ K2Fn p2_reduce(int mask) {
 hipError_t launch_p2d(const P2DJob& j, const LaunchCfg& c, hipStream_t s) {
   if (j.n_tgt <= 0) return hipSuccess;
   if (!p2d_supported(j.reg, j.mask) || j.n_src < 0 || c.wg != 256) return hipErrorInvalidValue;
+  // every output the mask selects must exist, and the inputs it reads
+  if (((j.mask & OUT_VEL) && !j.out_vel) || ((j.mask & OUT_VISC) && !j.out_visc) || !j.tgt ||
+      (j.n_src > 0 && !j.src))
+    return hipErrorInvalidValue;
   K2Fn k = p2_pick(j.reg, j.mask, j.acc, c.T);
   if (!k) return hipErrorInvalidValue;
   P2DKParams p{};

@@ -197,6 +197,10 @@ hipError_t launch_p3d(const P3DJob& j, const LaunchCfg& c, hipStream_t s) {
   if (j.n_src < 0 || c.wg != 256) return hipErrorInvalidValue;
   if ((j.mask & (OUT_DVORT | OUT_VISC)) && j.tgt_a == nullptr) return hipErrorInvalidValue;
   if (j.reg == REG_GAUSSIAN && j.gtab == nullptr) return hipErrorInvalidValue;
+  // every output the mask selects must exist, and the inputs it reads
+  if (((j.mask & OUT_VEL) && !j.out_vel) || ((j.mask & OUT_DVORT) && !j.out_dvort) ||
+      ((j.mask & OUT_VISC) && !j.out_visc) || !j.tgt_pv || (j.n_src > 0 && (!j.src_pv || !j.src_a)))
+    return hipErrorInvalidValue;
   KernelFn k = pick_kernel(j.reg, j.mask, j.acc, c.T);
   if (!k) return hipErrorInvalidValue;
 

@@ -69,6 +69,7 @@ int redistribute_host(const Rec** in, int n, Rec* out, int max_out, const cvtx_R
   if (n == 0) return 0;
   Device* d = active_device(rt);
   if (!d) return -1;
+  StreamUse use(*d, d->stream);
   const size_t pb = sizeof(float4) * (size_t)n;
   float4* h_pv = (float4*)host_buffer(rt, *d, SLOT_SRC0, pb);
   float4* d_pv = (float4*)device_buffer(rt, *d, SLOT_SRC0, pb);

@@ -77,6 +77,10 @@ static void release_device(Device& d) {
   }
   if (d.gtab) (void)hipFree(d.gtab);
   d.gtab = nullptr;
+  if (d.slot_event) (void)hipEventDestroy(d.slot_event);
+  d.slot_event = nullptr;
+  d.slot_stream = nullptr;
+  d.slot_pending = false;
   if (d.stream) (void)hipStreamDestroy(d.stream);
   d.stream = nullptr;
   d.ready = false;
@@ -86,12 +90,16 @@ void finalise_locked(Runtime& rt) {
   for (auto& d : rt.devices) release_device(d);
   rt.devices.clear();
   rt.initialised = false;
+  ++rt.generation;
 }
 
 static bool make_ready(Runtime& rt, Device& d) {
   if (d.ready) return true;
   if (!check_hip(rt, hipSetDevice(d.id), "hipSetDevice")) return false;
   if (!check_hip(rt, hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking), "hipStreamCreate"))
+    return false;
+  if (!check_hip(rt, hipEventCreateWithFlags(&d.slot_event, hipEventDisableTiming),
+                 "hipEventCreate"))
     return false;
   float tab[8 * GTAB_N];
   build_gaussian_table(tab);
@@ -150,6 +158,21 @@ void* host_buffer(Runtime& rt, Device& d, int slot, size_t bytes) {
   }
   b.bytes = want;
   return b.p;
+}
+
+StreamUse::StreamUse(Device& d, hipStream_t s) : d_(d), s_(s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  capturing_ = hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+  if (!capturing_ && d_.slot_pending && d_.slot_stream != s)
+    (void)hipStreamWaitEvent(s, d_.slot_event, 0);
+}
+
+StreamUse::~StreamUse() {
+  if (capturing_ || !d_.slot_event) return;
+  if (hipEventRecord(d_.slot_event, s_) == hipSuccess) {
+    d_.slot_stream = s_;
+    d_.slot_pending = true;
+  }
 }
 
 }  // namespace cvtx

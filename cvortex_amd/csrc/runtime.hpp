@@ -38,6 +38,9 @@ enum Slot : int {
   SLOT_COUNT
 };
 
+// Which leg the public void functions use (cvtx_hip_set_dispatch).
+enum Dispatch : int { DISPATCH_AUTO = 0, DISPATCH_DEVICE = 1, DISPATCH_HOST = 2 };
+
 struct Buffer {
   void* p = nullptr;
   size_t bytes = 0;
@@ -51,6 +54,12 @@ struct Device {
   int n_cu = 0;
   hipStream_t stream = nullptr;
   float4* gtab = nullptr;  // Gaussian near-field table
+  // The slot buffers are shared by every call on the device, whatever its
+  // stream: slot_event marks the end of the last call's use of them on
+  // slot_stream (see StreamUse).
+  hipEvent_t slot_event = nullptr;
+  hipStream_t slot_stream = nullptr;
+  bool slot_pending = false;
   Buffer dev[SLOT_COUNT];
   Buffer host[SLOT_COUNT];  // pinned
 };
@@ -62,6 +71,9 @@ struct Runtime {
   std::string last_error;
   int acc_mode = ACC_COMP;
   bool order_targets = true;
+  int dispatch = DISPATCH_AUTO;  // public void functions: which leg (dispatch.cpp)
+  int last_leg = -1;             // leg of the last public call (0 device, 1 host)
+  unsigned generation = 0;       // counts cvtx_finalise calls (objects made before are stale)
   std::recursive_mutex mu;
 };
 
@@ -80,6 +92,24 @@ void* device_buffer(Runtime& rt, Device& d, int slot, size_t bytes);
 void* host_buffer(Runtime& rt, Device& d, int slot, size_t bytes);
 
 void set_error(Runtime& rt, const std::string& msg);
+
+// Orders the calls that use a device's slot buffers across streams: made
+// at the start of a call on stream s, it makes s wait for the last such
+// call if that ran on another stream; at its end it marks this call's work
+// on s.  Nothing is recorded or waited for while s is being captured into
+// a graph (the graph orders its own nodes).
+class StreamUse {
+ public:
+  StreamUse(Device& d, hipStream_t s);
+  ~StreamUse();
+  StreamUse(const StreamUse&) = delete;
+  StreamUse& operator=(const StreamUse&) = delete;
+
+ private:
+  Device& d_;
+  hipStream_t s_;
+  bool capturing_ = false;
+};
 bool check_hip(Runtime& rt, hipError_t e, const char* what);
 
 // Run body(begin, end) over [0, n) on up to 16 host threads.

@@ -1,15 +1,17 @@
 // small_api.cpp -- the reference's single-source and single-target entry
-// points (S2S, S2M, M2S; reference libcvtx.h:125-211, 266-283, 306-360),
-// evaluated on the device through the M2M paths: S2M is a one-source M2M,
-// M2S a one-target M2M (whose source partitions fill the GPU), S2S a 1 x 1
-// M2M.  The values are those of the reference's pair and sum (the single
-// forms differ from M2M only in the final rounding of the one-term sum),
-// except for the vorticity field, whose single-source forms have no cut-off
-// and divide by the signed radius (reference src/P3D.cpp:146-161, 213-228).
+// points (S2S, S2M, M2S; reference libcvtx.h:125-211, 266-283, 306-360).
 //
-// Each call is one device round trip; many small evaluations are better
-// expressed as one M2M call.  On failure the result is NaN and a line goes
-// to stderr, as for the M2M entry points.
+// The reference evaluates these on the CPU only (src/P3D.cpp:74-296,
+// src/F3D.cpp:87-160, src/P2D.cpp:71-231).  Here a single pair (S2S) and
+// any S2M/M2S call with fewer than CVTX_HIP_SMALL_DEVICE_MIN elements run
+// on the library's CPU leg: one pair, or a few thousand, cost far less on
+// the host than a device round trip (gather, copies, launch, sync).  Larger
+// S2M calls are one-source M2M launches and larger M2S calls one-target M2M
+// launches, whose source partitions fill the GPU; the CPU leg remains their
+// fallback (dispatch.cpp).  The values are those of the reference's pair
+// and sum, except for the vorticity field, whose single-source forms have
+// no cut-off (reference src/P3D.cpp:146-161, 213-228).  Null records give
+// NaN results and a line on stderr (the reference asserts).
 #include <cmath>
 #include <limits>
 
@@ -33,27 +35,17 @@ bsv_V2f nan2() {
   return bsv_V2f{{q, q}};
 }
 
-// null single-record arguments are errors (the reference asserts)
-bool null_arg(const void* p) {
+// a null single record is an invalid argument: NaN results
+bool null_rec(const char* fn, const void* p, float* out, long n) {
   if (p) return false;
   set_error(runtime(), "null record");
+  report_failure(fn, out, n);
   return true;
 }
 
-bsv_V3f fail3(const char* fn) {
-  report_failure(fn, nullptr, 0);
-  return nan3();
-}
-
-bsv_V2f fail2(const char* fn) {
-  report_failure(fn, nullptr, 0);
-  return nan2();
-}
-
-float fail1(const char* fn) {
-  report_failure(fn, nullptr, 0);
-  return std::numeric_limits<float>::quiet_NaN();
-}
+// whether a single-source / single-target call of n elements is big enough
+// to be worth a device round trip
+bool big(int n) { return n >= CVTX_HIP_SMALL_DEVICE_MIN; }
 
 }  // namespace
 
@@ -63,195 +55,168 @@ extern "C" {
 
 bsv_V3f cvtx_P3D_S2S_vel(const cvtx_P3D* self, const bsv_V3f mes_point,
                          const cvtx_VortFunc* kernel, float regularisation_radius) {
-  if (null_arg(self)) return fail3("cvtx_P3D_S2S_vel");
+  bsv_V3f out = nan3();
+  if (null_rec("cvtx_P3D_S2S_vel", self, nullptr, 0)) return out;
   const cvtx_P3D* src[1] = {self};
-  bsv_V3f out;
-  if (cvtx_hip_P3D_M2M_vel(src, 1, &mes_point, 1, &out, kernel, regularisation_radius) != 0) {
-    report_failure("cvtx_P3D_S2S_vel", nullptr, 0);
-    return nan3();
-  }
+  run_p3d_vel("cvtx_P3D_S2S_vel", false, src, 1, &mes_point, 1, &out, kernel,
+              regularisation_radius);
   return out;
 }
 
 bsv_V3f cvtx_P3D_S2S_dvort(const cvtx_P3D* self, const cvtx_P3D* induced_particle,
                            const cvtx_VortFunc* kernel, float regularisation_radius) {
-  if (null_arg(self) || null_arg(induced_particle)) return fail3("cvtx_P3D_S2S_dvort");
+  bsv_V3f out = nan3();
+  if (null_rec("cvtx_P3D_S2S_dvort", self, nullptr, 0) ||
+      null_rec("cvtx_P3D_S2S_dvort", induced_particle, nullptr, 0))
+    return out;
   const cvtx_P3D* src[1] = {self};
   const cvtx_P3D* ind[1] = {induced_particle};
-  bsv_V3f out;
-  if (cvtx_hip_P3D_M2M_dvort(src, 1, ind, 1, &out, kernel, regularisation_radius) != 0) {
-    report_failure("cvtx_P3D_S2S_dvort", nullptr, 0);
-    return nan3();
-  }
+  run_p3d_dvort("cvtx_P3D_S2S_dvort", false, src, 1, ind, 1, &out, kernel,
+                regularisation_radius);
   return out;
 }
 
 bsv_V3f cvtx_P3D_S2S_visc_dvort(const cvtx_P3D* self, const cvtx_P3D* induced_particle,
                                 const cvtx_VortFunc* kernel, float regularisation_radius,
                                 float kinematic_visc) {
-  if (null_arg(self) || null_arg(induced_particle)) return fail3("cvtx_P3D_S2S_visc_dvort");
+  bsv_V3f out = nan3();
+  if (null_rec("cvtx_P3D_S2S_visc_dvort", self, nullptr, 0) ||
+      null_rec("cvtx_P3D_S2S_visc_dvort", induced_particle, nullptr, 0))
+    return out;
   const cvtx_P3D* src[1] = {self};
   const cvtx_P3D* ind[1] = {induced_particle};
-  bsv_V3f out;
-  if (cvtx_hip_P3D_M2M_visc_dvort(src, 1, ind, 1, &out, kernel, regularisation_radius,
-                                  kinematic_visc) != 0) {
-    report_failure("cvtx_P3D_S2S_visc_dvort", nullptr, 0);
-    return nan3();
-  }
+  run_p3d_visc("cvtx_P3D_S2S_visc_dvort", false, src, 1, ind, 1, &out, kernel,
+               regularisation_radius, kinematic_visc);
   return out;
 }
 
 bsv_V3f cvtx_P3D_S2S_vort(const cvtx_P3D* self, const bsv_V3f mes_point,
                           const cvtx_VortFunc* kernel, float regularisation_radius) {
-  if (null_arg(self)) return fail3("cvtx_P3D_S2S_vort");
+  bsv_V3f out = nan3();
+  if (null_rec("cvtx_P3D_S2S_vort", self, nullptr, 0)) return out;
   const cvtx_P3D* src[1] = {self};
-  bsv_V3f out;
-  if (vort_host(src, 1, &mes_point, 1, out.x, kernel, regularisation_radius, false) != 0) {
-    report_failure("cvtx_P3D_S2S_vort", nullptr, 0);
-    return nan3();
-  }
+  run_p3d_vort("cvtx_P3D_S2S_vort", false, src, 1, &mes_point, 1, &out, kernel,
+               regularisation_radius, false);
   return out;
 }
 
 void cvtx_P3D_S2M_vel(const cvtx_P3D* self, const bsv_V3f* mes_start, const int num_mes,
                       bsv_V3f* result_array, const cvtx_VortFunc* kernel,
                       float regularisation_radius) {
+  const long n = 3L * (num_mes > 0 ? num_mes : 0);
+  if (null_rec("cvtx_P3D_S2M_vel", self, (float*)result_array, n)) return;
   const cvtx_P3D* src[1] = {self};
-  if (null_arg(self) ||
-      cvtx_hip_P3D_M2M_vel(src, 1, mes_start, num_mes, result_array, kernel,
-                           regularisation_radius) != 0)
-    report_failure("cvtx_P3D_S2M_vel", (float*)result_array, 3L * (num_mes > 0 ? num_mes : 0));
+  run_p3d_vel("cvtx_P3D_S2M_vel", big(num_mes), src, 1, mes_start, num_mes, result_array, kernel,
+              regularisation_radius);
 }
 
 void cvtx_P3D_S2M_dvort(const cvtx_P3D* self, const cvtx_P3D** induced_start,
                         const int num_induced, bsv_V3f* result_array, const cvtx_VortFunc* kernel,
                         float regularisation_radius) {
+  const long n = 3L * (num_induced > 0 ? num_induced : 0);
+  if (null_rec("cvtx_P3D_S2M_dvort", self, (float*)result_array, n)) return;
   const cvtx_P3D* src[1] = {self};
-  if (null_arg(self) ||
-      cvtx_hip_P3D_M2M_dvort(src, 1, induced_start, num_induced, result_array, kernel,
-                             regularisation_radius) != 0)
-    report_failure("cvtx_P3D_S2M_dvort", (float*)result_array,
-                   3L * (num_induced > 0 ? num_induced : 0));
+  run_p3d_dvort("cvtx_P3D_S2M_dvort", big(num_induced), src, 1, induced_start, num_induced,
+                result_array, kernel, regularisation_radius);
 }
 
 void cvtx_P3D_S2M_visc_dvort(const cvtx_P3D* self, const cvtx_P3D** induced_start,
                              const int num_induced, bsv_V3f* result_array,
                              const cvtx_VortFunc* kernel, float regularisation_radius,
                              float kinematic_visc) {
+  const long n = 3L * (num_induced > 0 ? num_induced : 0);
+  if (null_rec("cvtx_P3D_S2M_visc_dvort", self, (float*)result_array, n)) return;
   const cvtx_P3D* src[1] = {self};
-  if (null_arg(self) ||
-      cvtx_hip_P3D_M2M_visc_dvort(src, 1, induced_start, num_induced, result_array, kernel,
-                                  regularisation_radius, kinematic_visc) != 0)
-    report_failure("cvtx_P3D_S2M_visc_dvort", (float*)result_array,
-                   3L * (num_induced > 0 ? num_induced : 0));
+  run_p3d_visc("cvtx_P3D_S2M_visc_dvort", big(num_induced), src, 1, induced_start, num_induced,
+               result_array, kernel, regularisation_radius, kinematic_visc);
 }
 
 void cvtx_P3D_S2M_vort(const cvtx_P3D* self, const bsv_V3f* mes_start, const int num_mes,
                        bsv_V3f* result_array, const cvtx_VortFunc* kernel,
                        float regularisation_radius) {
+  const long n = 3L * (num_mes > 0 ? num_mes : 0);
+  if (null_rec("cvtx_P3D_S2M_vort", self, (float*)result_array, n)) return;
   const cvtx_P3D* src[1] = {self};
-  if (null_arg(self) ||
-      vort_host(src, 1, mes_start, num_mes, (float*)result_array, kernel, regularisation_radius,
-                false) != 0)
-    report_failure("cvtx_P3D_S2M_vort", (float*)result_array, 3L * (num_mes > 0 ? num_mes : 0));
+  run_p3d_vort("cvtx_P3D_S2M_vort", big(num_mes), src, 1, mes_start, num_mes, result_array,
+               kernel, regularisation_radius, false);
 }
 
 bsv_V3f cvtx_P3D_M2S_vel(const cvtx_P3D** array_start, const int num_particles,
                          const bsv_V3f mes_point, const cvtx_VortFunc* kernel,
                          float regularisation_radius) {
-  bsv_V3f out;
-  if (cvtx_hip_P3D_M2M_vel(array_start, num_particles, &mes_point, 1, &out, kernel,
-                           regularisation_radius) != 0) {
-    report_failure("cvtx_P3D_M2S_vel", nullptr, 0);
-    return nan3();
-  }
+  bsv_V3f out = nan3();
+  run_p3d_vel("cvtx_P3D_M2S_vel", big(num_particles), array_start, num_particles, &mes_point, 1,
+              &out, kernel, regularisation_radius);
   return out;
 }
 
 bsv_V3f cvtx_P3D_M2S_dvort(const cvtx_P3D** array_start, const int num_particles,
                            const cvtx_P3D* induced_particle, const cvtx_VortFunc* kernel,
                            float regularisation_radius) {
-  if (null_arg(induced_particle)) return fail3("cvtx_P3D_M2S_dvort");
+  bsv_V3f out = nan3();
+  if (null_rec("cvtx_P3D_M2S_dvort", induced_particle, nullptr, 0)) return out;
   const cvtx_P3D* ind[1] = {induced_particle};
-  bsv_V3f out;
-  if (cvtx_hip_P3D_M2M_dvort(array_start, num_particles, ind, 1, &out, kernel,
-                             regularisation_radius) != 0) {
-    report_failure("cvtx_P3D_M2S_dvort", nullptr, 0);
-    return nan3();
-  }
+  run_p3d_dvort("cvtx_P3D_M2S_dvort", big(num_particles), array_start, num_particles, ind, 1,
+                &out, kernel, regularisation_radius);
   return out;
 }
 
 bsv_V3f cvtx_P3D_M2S_visc_dvort(const cvtx_P3D** array_start, const int num_particles,
                                 const cvtx_P3D* induced_particle, const cvtx_VortFunc* kernel,
                                 float regularisation_radius, float kinematic_visc) {
-  if (null_arg(induced_particle)) return fail3("cvtx_P3D_M2S_visc_dvort");
+  bsv_V3f out = nan3();
+  if (null_rec("cvtx_P3D_M2S_visc_dvort", induced_particle, nullptr, 0)) return out;
   const cvtx_P3D* ind[1] = {induced_particle};
-  bsv_V3f out;
-  if (cvtx_hip_P3D_M2M_visc_dvort(array_start, num_particles, ind, 1, &out, kernel,
-                                  regularisation_radius, kinematic_visc) != 0) {
-    report_failure("cvtx_P3D_M2S_visc_dvort", nullptr, 0);
-    return nan3();
-  }
+  run_p3d_visc("cvtx_P3D_M2S_visc_dvort", big(num_particles), array_start, num_particles, ind, 1,
+               &out, kernel, regularisation_radius, kinematic_visc);
   return out;
 }
 
 bsv_V3f cvtx_P3D_M2S_vort(const cvtx_P3D** array_start, const int num_particles,
                           const bsv_V3f mes_point, const cvtx_VortFunc* kernel,
                           float regularisation_radius) {
-  bsv_V3f out;
-  if (vort_host(array_start, num_particles, &mes_point, 1, out.x, kernel, regularisation_radius,
-                true) != 0) {
-    report_failure("cvtx_P3D_M2S_vort", nullptr, 0);
-    return nan3();
-  }
+  bsv_V3f out = nan3();
+  run_p3d_vort("cvtx_P3D_M2S_vort", big(num_particles), array_start, num_particles, &mes_point, 1,
+               &out, kernel, regularisation_radius, true);
   return out;
 }
 
 // ------------------------------- F3D -------------------------------
 
 bsv_V3f cvtx_F3D_S2S_vel(const cvtx_F3D* self, const bsv_V3f mes_point) {
-  if (null_arg(self)) return fail3("cvtx_F3D_S2S_vel");
+  bsv_V3f out = nan3();
+  if (null_rec("cvtx_F3D_S2S_vel", self, nullptr, 0)) return out;
   const cvtx_F3D* src[1] = {self};
-  bsv_V3f out;
-  if (cvtx_hip_F3D_M2M_vel(src, 1, &mes_point, 1, &out) != 0) {
-    report_failure("cvtx_F3D_S2S_vel", nullptr, 0);
-    return nan3();
-  }
+  run_f3d_vel("cvtx_F3D_S2S_vel", false, src, 1, &mes_point, 1, &out);
   return out;
 }
 
 bsv_V3f cvtx_F3D_S2S_dvort(const cvtx_F3D* self, const cvtx_P3D* induced_particle) {
-  if (null_arg(self) || null_arg(induced_particle)) return fail3("cvtx_F3D_S2S_dvort");
+  bsv_V3f out = nan3();
+  if (null_rec("cvtx_F3D_S2S_dvort", self, nullptr, 0) ||
+      null_rec("cvtx_F3D_S2S_dvort", induced_particle, nullptr, 0))
+    return out;
   const cvtx_F3D* src[1] = {self};
   const cvtx_P3D* ind[1] = {induced_particle};
-  bsv_V3f out;
-  if (cvtx_hip_F3D_M2M_dvort(src, 1, ind, 1, &out) != 0) {
-    report_failure("cvtx_F3D_S2S_dvort", nullptr, 0);
-    return nan3();
-  }
+  run_f3d_dvort("cvtx_F3D_S2S_dvort", false, src, 1, ind, 1, &out);
   return out;
 }
 
 bsv_V3f cvtx_F3D_M2S_vel(const cvtx_F3D** array_start, const int num_filaments,
                          const bsv_V3f mes_point) {
-  bsv_V3f out;
-  if (cvtx_hip_F3D_M2M_vel(array_start, num_filaments, &mes_point, 1, &out) != 0) {
-    report_failure("cvtx_F3D_M2S_vel", nullptr, 0);
-    return nan3();
-  }
+  bsv_V3f out = nan3();
+  run_f3d_vel("cvtx_F3D_M2S_vel", big(num_filaments), array_start, num_filaments, &mes_point, 1,
+              &out);
   return out;
 }
 
 bsv_V3f cvtx_F3D_M2S_dvort(const cvtx_F3D** array_start, const int num_filaments,
                            const cvtx_P3D* induced_particle) {
-  if (null_arg(induced_particle)) return fail3("cvtx_F3D_M2S_dvort");
+  bsv_V3f out = nan3();
+  if (null_rec("cvtx_F3D_M2S_dvort", induced_particle, nullptr, 0)) return out;
   const cvtx_P3D* ind[1] = {induced_particle};
-  bsv_V3f out;
-  if (cvtx_hip_F3D_M2M_dvort(array_start, num_filaments, ind, 1, &out) != 0) {
-    report_failure("cvtx_F3D_M2S_dvort", nullptr, 0);
-    return nan3();
-  }
+  run_f3d_dvort("cvtx_F3D_M2S_dvort", big(num_filaments), array_start, num_filaments, ind, 1,
+                &out);
   return out;
 }
 
@@ -259,50 +224,44 @@ bsv_V3f cvtx_F3D_M2S_dvort(const cvtx_F3D** array_start, const int num_filaments
 
 bsv_V2f cvtx_P2D_S2S_vel(const cvtx_P2D* self, const bsv_V2f mes_point,
                          const cvtx_VortFunc* kernel, float regularisation_radius) {
-  if (null_arg(self)) return fail2("cvtx_P2D_S2S_vel");
+  bsv_V2f out = nan2();
+  if (null_rec("cvtx_P2D_S2S_vel", self, nullptr, 0)) return out;
   const cvtx_P2D* src[1] = {self};
-  bsv_V2f out;
-  if (cvtx_hip_P2D_M2M_vel(src, 1, &mes_point, 1, &out, kernel, regularisation_radius) != 0) {
-    report_failure("cvtx_P2D_S2S_vel", nullptr, 0);
-    return nan2();
-  }
+  run_p2d_vel("cvtx_P2D_S2S_vel", false, src, 1, &mes_point, 1, &out, kernel,
+              regularisation_radius);
   return out;
 }
 
 void cvtx_P2D_S2M_vel(const cvtx_P2D* self, const bsv_V2f* mes_start, const int num_mes,
                       bsv_V2f* result_array, const cvtx_VortFunc* kernel,
                       float regularisation_radius) {
+  const long n = 2L * (num_mes > 0 ? num_mes : 0);
+  if (null_rec("cvtx_P2D_S2M_vel", self, (float*)result_array, n)) return;
   const cvtx_P2D* src[1] = {self};
-  if (null_arg(self) ||
-      cvtx_hip_P2D_M2M_vel(src, 1, mes_start, num_mes, result_array, kernel,
-                           regularisation_radius) != 0)
-    report_failure("cvtx_P2D_S2M_vel", (float*)result_array, 2L * (num_mes > 0 ? num_mes : 0));
+  run_p2d_vel("cvtx_P2D_S2M_vel", big(num_mes), src, 1, mes_start, num_mes, result_array, kernel,
+              regularisation_radius);
 }
 
 bsv_V2f cvtx_P2D_M2S_vel(const cvtx_P2D** array_start, const int num_particles,
                          const bsv_V2f mes_point, const cvtx_VortFunc* kernel,
                          float regularisation_radius) {
-  bsv_V2f out;
-  if (cvtx_hip_P2D_M2M_vel(array_start, num_particles, &mes_point, 1, &out, kernel,
-                           regularisation_radius) != 0) {
-    report_failure("cvtx_P2D_M2S_vel", nullptr, 0);
-    return nan2();
-  }
+  bsv_V2f out = nan2();
+  run_p2d_vel("cvtx_P2D_M2S_vel", big(num_particles), array_start, num_particles, &mes_point, 1,
+              &out, kernel, regularisation_radius);
   return out;
 }
 
 float cvtx_P2D_S2S_visc_dvort(const cvtx_P2D* self, const cvtx_P2D* induced_particle,
                               const cvtx_VortFunc* kernel, float regularisation_radius,
                               float kinematic_visc) {
-  if (null_arg(self) || null_arg(induced_particle)) return fail1("cvtx_P2D_S2S_visc_dvort");
+  float out = std::numeric_limits<float>::quiet_NaN();
+  if (null_rec("cvtx_P2D_S2S_visc_dvort", self, nullptr, 0) ||
+      null_rec("cvtx_P2D_S2S_visc_dvort", induced_particle, nullptr, 0))
+    return out;
   const cvtx_P2D* src[1] = {self};
   const cvtx_P2D* ind[1] = {induced_particle};
-  float out;
-  if (cvtx_hip_P2D_M2M_visc_dvort(src, 1, ind, 1, &out, kernel, regularisation_radius,
-                                  kinematic_visc) != 0) {
-    report_failure("cvtx_P2D_S2S_visc_dvort", nullptr, 0);
-    return std::numeric_limits<float>::quiet_NaN();
-  }
+  run_p2d_visc("cvtx_P2D_S2S_visc_dvort", false, src, 1, ind, 1, &out, kernel,
+               regularisation_radius, kinematic_visc);
   return out;
 }
 
@@ -310,25 +269,21 @@ void cvtx_P2D_S2M_visc_dvort(const cvtx_P2D* self, const cvtx_P2D** induced_star
                              const int num_induced, float* result_array,
                              const cvtx_VortFunc* kernel, float regularisation_radius,
                              float kinematic_visc) {
+  const long n = num_induced > 0 ? num_induced : 0;
+  if (null_rec("cvtx_P2D_S2M_visc_dvort", self, result_array, n)) return;
   const cvtx_P2D* src[1] = {self};
-  if (null_arg(self) ||
-      cvtx_hip_P2D_M2M_visc_dvort(src, 1, induced_start, num_induced, result_array, kernel,
-                                  regularisation_radius, kinematic_visc) != 0)
-    report_failure("cvtx_P2D_S2M_visc_dvort", result_array,
-                   (long)(num_induced > 0 ? num_induced : 0));
+  run_p2d_visc("cvtx_P2D_S2M_visc_dvort", big(num_induced), src, 1, induced_start, num_induced,
+               result_array, kernel, regularisation_radius, kinematic_visc);
 }
 
 float cvtx_P2D_M2S_visc_dvort(const cvtx_P2D** array_start, const int num_particles,
                               const cvtx_P2D* induced_particle, const cvtx_VortFunc* kernel,
                               float regularisation_radius, float kinematic_visc) {
-  if (null_arg(induced_particle)) return fail1("cvtx_P2D_M2S_visc_dvort");
+  float out = std::numeric_limits<float>::quiet_NaN();
+  if (null_rec("cvtx_P2D_M2S_visc_dvort", induced_particle, nullptr, 0)) return out;
   const cvtx_P2D* ind[1] = {induced_particle};
-  float out;
-  if (cvtx_hip_P2D_M2M_visc_dvort(array_start, num_particles, ind, 1, &out, kernel,
-                                  regularisation_radius, kinematic_visc) != 0) {
-    report_failure("cvtx_P2D_M2S_visc_dvort", nullptr, 0);
-    return std::numeric_limits<float>::quiet_NaN();
-  }
+  run_p2d_visc("cvtx_P2D_M2S_visc_dvort", big(num_particles), array_start, num_particles, ind, 1,
+               &out, kernel, regularisation_radius, kinematic_visc);
   return out;
 }
 

@@ -167,6 +167,7 @@ int cvtx_hip_autotune_p3d(const cvtx_hip_p3d_problem* p, int reps, int* best_T, 
   if (!p || reps < 1) return -1;
   Device* d = active_device(rt);
   if (!d) return -1;
+  StreamUse use(*d, d->stream);
   P3DJob j;
   j.src_pv = (const float4*)p->src_pv;
   j.src_a = (const float4*)p->src_a;

@@ -78,6 +78,7 @@ int cvtx::vort_host(const cvtx_P3D** src, int ns, const bsv_V3f* mes, int nt, fl
   }
   Device* d = active_device(rt);
   if (!d) return -1;
+  StreamUse use(*d, d->stream);
   const size_t sb = sizeof(float4) * (size_t)(ns > 0 ? ns : 1);
   const size_t tb = sizeof(float4) * (size_t)nt;
   const size_t ob = sizeof(float) * 3 * (size_t)nt;
@@ -115,6 +116,7 @@ int relax_host(cvtx_P3D** p, int n, float fdt, const cvtx_VortFunc* vf, float si
   if (n == 0) return 0;
   Device* d = active_device(rt);
   if (!d) return -1;
+  StreamUse use(*d, d->stream);
   const size_t pb = sizeof(float4) * (size_t)n;
   const size_t ob = sizeof(float) * 3 * (size_t)n;
   float4* h_pv = (float4*)host_buffer(rt, *d, SLOT_SRC0, pb);
@@ -156,25 +158,6 @@ int cvtx_hip_P3D_pedrizzetti_relaxation(cvtx_P3D** particles, int n, float fdt,
   return relax_host(particles, n, fdt, k, sigma);
 }
 
-// ============================= public API =============================
-// reference src/P3D.cpp:476-498 and :669-707
-
-void cvtx_P3D_M2M_vort(const cvtx_P3D** src, const int ns, const bsv_V3f* mes, const int nm,
-                       bsv_V3f* out, const cvtx_VortFunc* k, float sigma) {
-  if (cvtx_hip_P3D_M2M_vort(src, ns, mes, nm, out, k, sigma) != 0)
-    report_failure("cvtx_P3D_M2M_vort", (float*)out, 3L * (nm > 0 ? nm : 0));
-}
-
-void cvtx_P3D_pedrizzetti_relaxation(cvtx_P3D** particles, const int n, float fdt,
-                                     const cvtx_VortFunc* k, float sigma) {
-  if (cvtx_hip_P3D_pedrizzetti_relaxation(particles, n, fdt, k, sigma) != 0) {
-    // the strengths are the result here: report and set them to NaN
-    report_failure("cvtx_P3D_pedrizzetti_relaxation", nullptr, 0);
-    for (int i = 0; particles && i < n; ++i)
-      if (particles[i]) fill_nan(particles[i]->vorticity.x, 3);
-  }
-}
-
 // ========================= device-resident API =========================
 
 static int vort_dev(const cvtx_hip_vort_problem* p, const LaunchCfg* forced, void* stream) {
@@ -189,6 +172,7 @@ static int vort_dev(const cvtx_hip_vort_problem* p, const LaunchCfg* forced, voi
   }
   Device* d = active_device(rt);
   if (!d) return -1;
+  StreamUse use(*d, pick_stream(d, stream));
   const bool same = p->src_pv == p->tgt && p->n_src == p->n_tgt;
   return vort_core(rt, *d, (const float4*)p->src_pv, (const float4*)p->src_a, p->n_src,
                    (const float4*)p->tgt, p->n_tgt, p->vort, p->regularisation, p->accumulate,
@@ -222,6 +206,7 @@ int cvtx_hip_relax_eval(const float* pv, float* a, int n, float fdt, int regular
   if (n == 0) return 0;
   Device* d = active_device(rt);
   if (!d) return -1;
+  StreamUse use(*d, pick_stream(d, stream));
   float* om = (float*)device_buffer(rt, *d, SLOT_OUT2, sizeof(float) * 3 * (size_t)n);
   if (!om) return -1;
   const hipStream_t s = pick_stream(d, stream);

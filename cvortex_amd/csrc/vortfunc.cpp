@@ -7,6 +7,7 @@
 // the GPU kernels (whose Gaussian uses exact erf values) rather than with the
 // reference's FP32 Abramowitz & Stegun approximation, which loses accuracy
 // below rho ~ 0.5.  Formulas: reference src/VortFunc.cpp:64-199.
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -18,14 +19,13 @@ namespace {
 constexpr double kSqrt2OverPi = 0.79788456080286535588;
 
 float eta_missing(float) {
-  // reference src/VortFunc.cpp:52-62: one warning, then zero
-  static bool warned = false;
-  if (!warned) {
+  // reference src/VortFunc.cpp:52-62: one warning, then zero (the CPU leg
+  // calls this from several threads at once)
+  static std::atomic<bool> warned{false};
+  if (!warned.exchange(true))
     std::fprintf(stderr,
                  "cvortex: eta requested from a regularisation without a viscous kernel "
                  "(singular or planetary); returning zero.\n");
-    warned = true;
-  }
   return 0.f;
 }
 

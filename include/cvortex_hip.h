@@ -10,16 +10,27 @@
  *    first enabled HIP device, and return 0 on success or -1 on failure
  *    (no device enabled, unsupported regularisation name, invalid argument,
  *    HIP error); cvtx_hip_last_error() then describes the failure.  They
- *    never run a CPU fallback themselves.
+ *    never run a CPU fallback themselves; the public void functions do
+ *    (see cvtx_hip_set_dispatch).
  *
  * 2. Fused host-array entry points (no reference counterpart): one pass
  *    over the pairs produces several outputs.
  *
  * 3. The device-resident API: structure-of-arrays device buffers and a
  *    caller stream, for callers that keep particles in GPU memory (time
- *    stepping, batched clouds, graph capture).  All pointers are device
- *    pointers; `stream` is a hipStream_t passed as void* (NULL = the
- *    library's stream for the device).  Calls are asynchronous.
+ *    stepping, batched clouds).  All pointers are device pointers; `stream`
+ *    is a hipStream_t passed as void* (NULL = the library's stream for the
+ *    device).  Calls are asynchronous.  Calls that need workspace (source
+ *    partials, target ordering) share one grow-only workspace per device:
+ *    a call on one stream waits (through an event) for the last such call
+ *    on any other stream, so calls on different streams do not overlap.
+ *    A call captured into a graph keeps the workspace address it saw, and
+ *    a later call that needs more workspace replaces that buffer; graphs
+ *    should therefore be captured around a size that an earlier eager call
+ *    has already reached, and replayed while no larger call intervenes.
+ *    Capturing the first call of a size fails (the growth synchronises).
+ *    The library's own RK4 graphs (cvtx_hip_batch_rk4_*) use buffers of
+ *    their batch only.
  *
  * Regularisation ids follow the reference's kernel names
  * (cl_kernel_name_ext, reference src/VortFunc.cpp:210,223,236,249).
@@ -119,6 +130,35 @@ int cvtx_hip_P2D_M2M_fused(const cvtx_P2D** array_start, int num_particles,
 
 /* ---- controls (no reference counterpart) ---- */
 const char* cvtx_hip_last_error(void);
+
+/* Which leg evaluates the public void cvtx_* functions.
+ * AUTO (default): the device when one is enabled, the regularisation's
+ *   cl_kernel_name_ext is one of the four kernel names, and the problem
+ *   passes the reference's size guards (both counts >= 256 for P3D, F3D
+ *   dvort and P2D visc; the reference's cost models for F3D and P2D vel,
+ *   src/ocl_F3D.cpp:49-51, src/ocl_P2D.cpp:54-57); the single-source and
+ *   single-target forms go to the device from CVTX_HIP_SMALL_DEVICE_MIN
+ *   elements on.  Otherwise, or when the device call fails, the library's
+ *   CPU leg: the caller's cvtx_VortFunc function pointers per pair, FP64
+ *   sums, host threads (CVTX_NUM_THREADS or OMP_NUM_THREADS).
+ * DEVICE: every call the device can take goes to it (the CPU leg remains
+ *   for no device, an unknown kernel name, or a failure).
+ * HOST: every call runs on the CPU leg.
+ * Returns the previous mode, or -1 for an invalid mode.  The int seam
+ * (cvtx_hip_*) never uses the CPU leg. */
+#define CVTX_HIP_DISPATCH_AUTO 0
+#define CVTX_HIP_DISPATCH_DEVICE 1
+#define CVTX_HIP_DISPATCH_HOST 2
+#define CVTX_HIP_SMALL_DEVICE_MIN 65536
+int cvtx_hip_set_dispatch(int mode);
+int cvtx_hip_get_dispatch(void);
+/* Leg that evaluated the last public void call: CVTX_HIP_LEG_DEVICE,
+ * CVTX_HIP_LEG_HOST, or -1 before any call (or after a failed one). */
+#define CVTX_HIP_LEG_DEVICE 0
+#define CVTX_HIP_LEG_HOST 1
+int cvtx_hip_last_leg(void);
+/* Host threads the CPU leg uses. */
+int cvtx_hip_host_threads(void);
 /* Accumulation used by every call that does not name one; returns the
  * previous mode, or -1 for an invalid mode. Default CVTX_HIP_ACC_COMP. */
 int cvtx_hip_set_accumulation(int mode);

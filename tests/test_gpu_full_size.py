@@ -43,18 +43,29 @@ def rms_rel(got, ref):
 
 
 def test_ring_2pow20(orc):
-    p, sigma, _ = W.ring()
+    p, sigma, nu = W.ring()
     n = len(p)
-    case(f"ring n={n}: fused vel+dvort, Gaussian")
-    v, d, _ = cv.P3D_M2M_fused(p, p, "gaussian", sigma, 0.0, vel=True, dvort=True)
+    case(f"ring n={n}: fused vel+dvort+visc, Gaussian, nu={nu}")
+    v, d, e3 = cv.P3D_M2M_fused(p, p, "gaussian", sigma, nu, vel=True, dvort=True, visc=True)
     idx = sample(n, 256, 1)
-    v64, d64, _ = orc.f64_P3D(p, p, "gaussian", sigma, idx=idx, gauss_mode=1)
+    v64, d64, e64 = orc.f64_P3D(p, p, "gaussian", sigma, nu, idx=idx, gauss_mode=1, visc=True)
     for name, g, r in (("vel", v[idx], v64), ("dvort", d[idx], d64)):
         assert np.all(np.isfinite(g))
         e, rms = oracle.rel_l2(g, r), rms_rel(g, r)
         print(f"  {name}: relL2 vs FP64 {e:.2e}, RMS per target {rms:.2e}", flush=True)
         assert e <= TOL
         assert rms <= 1e-6, f"{name}: RMS relative error {rms:.2e} > 1e-6"
+    # the viscous exchange is a difference of nearly equal strengths summed
+    # over the core: relative L2 <= 1e-5 against FP64 on the same targets
+    g = e3[idx]
+    assert np.all(np.isfinite(g))
+    e, rms = oracle.rel_l2(g, e64), rms_rel(g, e64)
+    print(f"  visc: relL2 vs FP64 {e:.2e}, RMS per target {rms:.2e}", flush=True)
+    assert e <= TOL, f"visc: relative L2 {e:.2e} > {TOL}"
+    case("ring: vel and dvort of the fused call with visc equal the call without it")
+    v1, d1, _ = cv.P3D_M2M_fused(p, p, "gaussian", sigma, 0.0, vel=True, dvort=True)
+    assert np.array_equal(v1, v) and np.array_equal(d1, d)
+    v, d = v1, d1
     case("ring: negative sigma negates dvort exactly, vel unchanged")
     v2, d2, _ = cv.P3D_M2M_fused(p, p, "gaussian", -sigma, 0.0, vel=True, dvort=True)
     assert np.array_equal(v2, v)

@@ -3,7 +3,9 @@ against the oracle's restatement of the reference CPU path (reference
 src/P3D.cpp:74-296, src/F3D.cpp:34-128, src/P2D.cpp:71-231).  Tolerance:
 relative L2 <= 1e-5 per call (the Gaussian stretching term, whose FP32
 reference value carries the reference's own erf error, is checked with the
-FP64-limited policy)."""
+FP64-limited policy).  The dispatch mode DEVICE sends these small calls to
+the device (in the automatic mode they run on the CPU leg, which
+tests/test_host_leg.py covers); each test checks that the device ran."""
 import numpy as np
 import pytest
 
@@ -22,6 +24,15 @@ def gpu():
     if not have_gpu():
         pytest.fail("no HIP device: GPU tests need one")
     cv.set_accumulation(cv.ACC_COMP)
+    prev = cv.set_dispatch(cv.DISPATCH_DEVICE)
+    yield
+    cv.set_dispatch(prev)
+
+
+@pytest.fixture(autouse=True)
+def on_device():
+    yield
+    assert cv.last_leg() == cv.LEG_DEVICE, "the last call did not run on the device"
 
 
 @pytest.fixture(scope="module")

@@ -1,6 +1,6 @@
 """libcvortex.so: loads, exports every symbol the headers declare, has the
 reference's struct layouts, and behaves like the reference's controls.
-These tests make no compute call and run without a GPU."""
+These tests make no device compute call and run without a GPU."""
 import ctypes
 import os
 import re
@@ -139,38 +139,30 @@ def test_unknown_regularisation_is_an_error(L):
         cv.P3D_M2M_vel(p, np.zeros((4, 3), np.float32), vf, 0.1)
 
 
-def test_no_device_public_api_reports_nan(L):
-    # With every accelerator disabled the public void API fills NaN: it never
-    # silently computes on the CPU.
-    cv.initialise()
-    n = cv.num_accelerators()
-    states = [cv.accelerator_enabled(i) for i in range(n)]
-    for i in range(n):
-        cv.accelerator_disable(i)
-    try:
-        p = np.zeros(3, P3D_DTYPE)
-        p["coord"] = np.arange(9, dtype=np.float32).reshape(3, 3)
-        from cvortex_amd.types import pointer_array
-        ptrs, keep = pointer_array(p)
-        out = np.zeros((3, 3), np.float32)
-        vf = cv.vortfunc("singular")
-        mes = np.ascontiguousarray(p["coord"])
-        L.cvtx_P3D_M2M_vel(ctypes.c_void_p(ptrs.ctypes.data), 3, ctypes.c_void_p(mes.ctypes.data), 3,
-                           ctypes.c_void_p(out.ctypes.data), ctypes.byref(vf), 0.1)
-        assert np.isnan(out).all()
-        with pytest.raises(cv.CvtxError):
-            cv.P3D_M2M_vel(p, p["coord"], "singular", 0.1)
-        out[:] = 0
-        L.cvtx_P3D_M2M_vort(ctypes.c_void_p(ptrs.ctypes.data), 3, ctypes.c_void_p(mes.ctypes.data), 3,
-                            ctypes.c_void_p(out.ctypes.data), ctypes.byref(vf), 0.1)
-        assert np.isnan(out).all()
-        # the relaxation's result is the strengths themselves
-        L.cvtx_P3D_pedrizzetti_relaxation(ctypes.c_void_p(ptrs.ctypes.data), 3, 0.1,
-                                          ctypes.byref(vf), 0.1)
-        assert np.isnan(p["vorticity"]).all()
-    finally:
-        for i, st in enumerate(states):
-            (cv.accelerator_enable if st else cv.accelerator_disable)(i)
+def test_public_api_invalid_arguments_report_nan(L):
+    # The public void API has no error return: invalid arguments (here a
+    # null regularisation) give NaN results and a line on stderr, on either
+    # leg, while the seam reports -1.  (With no device enabled the public
+    # functions compute on the CPU leg: tests/test_host_leg.py.)
+    p = np.zeros(3, P3D_DTYPE)
+    p["coord"] = np.arange(9, dtype=np.float32).reshape(3, 3)
+    from cvortex_amd.types import pointer_array
+    ptrs, keep = pointer_array(p)
+    out = np.zeros((3, 3), np.float32)
+    mes = np.ascontiguousarray(p["coord"])
+    for mode in (cv.DISPATCH_HOST, cv.DISPATCH_DEVICE):
+        prev = cv.set_dispatch(mode)
+        try:
+            out[:] = 0
+            L.cvtx_P3D_M2M_vel(ctypes.c_void_p(ptrs.ctypes.data), 3, ctypes.c_void_p(mes.ctypes.data),
+                               3, ctypes.c_void_p(out.ctypes.data), None, ctypes.c_float(0.1))
+            assert np.isnan(out).all()
+            assert cv.last_leg() == -1
+        finally:
+            cv.set_dispatch(prev)
+    assert L.cvtx_hip_P3D_M2M_vel(ctypes.c_void_p(ptrs.ctypes.data), 3,
+                                  ctypes.c_void_p(mes.ctypes.data), 3,
+                                  ctypes.c_void_p(out.ctypes.data), None, ctypes.c_float(0.1)) == -1
 
 
 def test_redistribution_functions_match_oracle(L, orc):
