@@ -133,6 +133,7 @@ _SIGNATURES = {
     "cvtx_hip_p2d_eval": (_I, [_VP, _VP]),
     "cvtx_hip_f3d_eval": (_I, [_VP, _VP]),
     "cvtx_hip_p3d_eval_shape": (_I, [_VP, _I, _I, _VP]),
+    "cvtx_hip_p3d_eval_shape_noalloc": (_I, [_VP, _I, _I, _VP]),
     "cvtx_hip_p2d_eval_shape": (_I, [_VP, _I, _I, _VP]),
     "cvtx_hip_f3d_eval_shape": (_I, [_VP, _I, _I, _VP]),
     "cvtx_hip_autotune_p3d": (_I, [_VP, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),

@@ -399,6 +399,7 @@ void cvtx_initialise() {
   Runtime& rt = runtime();
   Lock lk(rt.mu);
   initialise_locked(rt);
+  ready_enabled_devices(rt);
 }
 
 void cvtx_finalise() {
@@ -451,6 +452,7 @@ void cvtx_accelerator_enable(int id) {
   Lock lk(rt.mu);
   initialise_locked(rt);
   if (id >= 0 && id < (int)rt.devices.size()) rt.devices[id].enabled = true;
+  ready_enabled_devices(rt);
 }
 
 void cvtx_accelerator_disable(int id) {
@@ -560,16 +562,15 @@ int cvtx_hip_synchronize(void) {
 
 // ========================= device-resident API =========================
 
-static int p3d_dev(const cvtx_hip_p3d_problem* p, const LaunchCfg* forced, void* stream) {
-  Runtime& rt = runtime();
-  Lock lk(rt.mu);
+}  // extern "C"
+
+namespace cvtx {
+
+bool p3d_job_from_problem(Runtime& rt, const Device& d, const cvtx_hip_p3d_problem* p, P3DJob* out) {
   if (!p) {
     set_error(rt, "null problem");
-    return -1;
+    return false;
   }
-  Device* d = active_device(rt);
-  if (!d) return -1;
-  StreamUse use(*d, pick_stream(d, stream));
   P3DJob j;
   j.src_pv = (const float4*)p->src_pv;
   j.src_a = (const float4*)p->src_a;
@@ -585,20 +586,36 @@ static int p3d_dev(const cvtx_hip_p3d_problem* p, const LaunchCfg* forced, void*
   j.acc = p->accumulate;
   j.sigma = p->sigma;
   j.nu = p->nu;
-  j.gtab = d->gtab;
+  j.gtab = d.gtab;
   if (j.n_src < 0 || j.n_tgt < 0 || (j.n_src > 0 && (!j.src_pv || !j.src_a)) ||
       (j.n_tgt > 0 && !j.tgt_pv)) {
     set_error(rt, "invalid device problem");
-    return -1;
+    return false;
   }
   if (j.acc != ACC_PLAIN && j.acc != ACC_COMP) {
     set_error(rt, "invalid accumulation mode");
-    return -1;
+    return false;
   }
   if (!p3d_supported(j.reg, j.mask)) {
     set_error(rt, "unsupported regularisation/output combination");
-    return -1;
+    return false;
   }
+  *out = j;
+  return true;
+}
+
+}  // namespace cvtx
+
+extern "C" {
+
+static int p3d_dev(const cvtx_hip_p3d_problem* p, const LaunchCfg* forced, void* stream) {
+  Runtime& rt = runtime();
+  Lock lk(rt.mu);
+  Device* d = active_device(rt);
+  if (!d) return -1;
+  P3DJob j;
+  if (!p3d_job_from_problem(rt, *d, p, &j)) return -1;
+  StreamUse use(*d, pick_stream(d, stream));
   LaunchCfg c = forced ? *forced : choose_cfg(KIND_P3D, j.reg, j.mask, j.acc, j.n_src, j.n_tgt, d->n_cu);
   const size_t wb = p3d_workspace_bytes(j, c);
   if (wb) {

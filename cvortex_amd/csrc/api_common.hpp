@@ -9,6 +9,7 @@
 #include <string>
 
 #include <cvortex/libcvtx.h>
+#include <cvortex_hip.h>
 
 #include "kernels.hpp"
 #include "runtime.hpp"
@@ -75,6 +76,9 @@ void run_p2d_vel(const char* fn, bool guard, const cvtx_P2D** src, int ns, const
 void run_p2d_visc(const char* fn, bool guard, const cvtx_P2D** src, int ns,
                   const cvtx_P2D** ind, int ni, float* out, const cvtx_VortFunc* k, float sigma,
                   float nu);
+// A device-resident problem as a P3D job on device d (validated; false and
+// the error recorded when it is invalid).
+bool p3d_job_from_problem(Runtime& rt, const Device& d, const cvtx_hip_p3d_problem* p, P3DJob* j);
 // explicit launch shape: T in {1, 2, 4}, n_split >= 1 source partitions
 // (rounded to whole 256-source tiles); false for an invalid shape
 bool make_launch_shape(int T, int n_split, int n_src, LaunchCfg* c);

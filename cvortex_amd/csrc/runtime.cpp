@@ -77,10 +77,12 @@ static void release_device(Device& d) {
   }
   if (d.gtab) (void)hipFree(d.gtab);
   d.gtab = nullptr;
-  if (d.slot_event) (void)hipEventDestroy(d.slot_event);
-  d.slot_event = nullptr;
-  d.slot_stream = nullptr;
-  d.slot_pending = false;
+  for (UseOrder* o : {&d.slot_use, &d.scratch_use}) {
+    if (o->ev) (void)hipEventDestroy(o->ev);
+    *o = UseOrder{};
+  }
+  d.scratch = nullptr;
+  d.scratch_bytes = 0;
   if (d.stream) (void)hipStreamDestroy(d.stream);
   d.stream = nullptr;
   d.ready = false;
@@ -98,9 +100,10 @@ static bool make_ready(Runtime& rt, Device& d) {
   if (!check_hip(rt, hipSetDevice(d.id), "hipSetDevice")) return false;
   if (!check_hip(rt, hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking), "hipStreamCreate"))
     return false;
-  if (!check_hip(rt, hipEventCreateWithFlags(&d.slot_event, hipEventDisableTiming),
-                 "hipEventCreate"))
-    return false;
+  for (UseOrder* o : {&d.slot_use, &d.scratch_use})
+    if (!check_hip(rt, hipEventCreateWithFlags(&o->ev, hipEventDisableTiming), "hipEventCreate"))
+      return false;
+  if (!attach_scratch(rt, d)) return false;
   float tab[8 * GTAB_N];
   build_gaussian_table(tab);
   if (!check_hip(rt, hipMalloc(&d.gtab, sizeof(tab)), "hipMalloc(table)")) return false;
@@ -160,19 +163,23 @@ void* host_buffer(Runtime& rt, Device& d, int slot, size_t bytes) {
   return b.p;
 }
 
-StreamUse::StreamUse(Device& d, hipStream_t s) : d_(d), s_(s) {
+StreamUse::StreamUse(UseOrder& o, hipStream_t s) : o_(o), s_(s) {
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   capturing_ = hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
-  if (!capturing_ && d_.slot_pending && d_.slot_stream != s)
-    (void)hipStreamWaitEvent(s, d_.slot_event, 0);
+  if (!capturing_ && o_.pending && o_.stream != s) (void)hipStreamWaitEvent(s, o_.ev, 0);
 }
 
 StreamUse::~StreamUse() {
-  if (capturing_ || !d_.slot_event) return;
-  if (hipEventRecord(d_.slot_event, s_) == hipSuccess) {
-    d_.slot_stream = s_;
-    d_.slot_pending = true;
+  if (capturing_ || !o_.ev) return;
+  if (hipEventRecord(o_.ev, s_) == hipSuccess) {
+    o_.stream = s_;
+    o_.pending = true;
   }
+}
+
+void ready_enabled_devices(Runtime& rt) {
+  for (auto& d : rt.devices)
+    if (d.enabled) (void)make_ready(rt, d);
 }
 
 }  // namespace cvtx

@@ -46,6 +46,14 @@ struct Buffer {
   size_t bytes = 0;
 };
 
+// The last use of a shared device buffer: an event recorded on `stream`
+// after that use (pending = recorded and not yet superseded).
+struct UseOrder {
+  hipEvent_t ev = nullptr;
+  hipStream_t stream = nullptr;
+  bool pending = false;
+};
+
 struct Device {
   int id = -1;
   std::string name;
@@ -55,11 +63,13 @@ struct Device {
   hipStream_t stream = nullptr;
   float4* gtab = nullptr;  // Gaussian near-field table
   // The slot buffers are shared by every call on the device, whatever its
-  // stream: slot_event marks the end of the last call's use of them on
-  // slot_stream (see StreamUse).
-  hipEvent_t slot_event = nullptr;
-  hipStream_t slot_stream = nullptr;
-  bool slot_pending = false;
+  // stream; so is the fixed scratch array of the no-allocation entry point
+  // (noalloc.hip).  Each UseOrder marks the end of the last call's use of
+  // one of them (see StreamUse).
+  UseOrder slot_use;
+  UseOrder scratch_use;
+  void* scratch = nullptr;  // this device's address of the scratch array
+  size_t scratch_bytes = 0;
   Buffer dev[SLOT_COUNT];
   Buffer host[SLOT_COUNT];  // pinned
 };
@@ -93,23 +103,32 @@ void* host_buffer(Runtime& rt, Device& d, int slot, size_t bytes);
 
 void set_error(Runtime& rt, const std::string& msg);
 
-// Orders the calls that use a device's slot buffers across streams: made
-// at the start of a call on stream s, it makes s wait for the last such
-// call if that ran on another stream; at its end it marks this call's work
-// on s.  Nothing is recorded or waited for while s is being captured into
-// a graph (the graph orders its own nodes).
+// Orders the calls that use a shared device buffer across streams: made at
+// the start of a call on stream s, it makes s wait for the last such call
+// if that ran on another stream; at its end it marks this call's work on s.
+// Nothing is recorded or waited for while s is being captured into a graph
+// (the graph orders its own nodes).  StreamUse(d, s) guards the slot
+// buffers.
 class StreamUse {
  public:
-  StreamUse(Device& d, hipStream_t s);
+  StreamUse(UseOrder& o, hipStream_t s);
+  StreamUse(Device& d, hipStream_t s) : StreamUse(d.slot_use, s) {}
   ~StreamUse();
   StreamUse(const StreamUse&) = delete;
   StreamUse& operator=(const StreamUse&) = delete;
 
  private:
-  Device& d_;
+  UseOrder& o_;
   hipStream_t s_;
   bool capturing_ = false;
 };
+
+// Looks up this device's address of the no-allocation scratch array
+// (noalloc.hip); called with the device current when it is made ready.
+bool attach_scratch(Runtime& rt, Device& d);
+// Makes every enabled device ready (stream, table, scratch address): the
+// set-up cvtx_initialise does so that later calls need not.
+void ready_enabled_devices(Runtime& rt);
 bool check_hip(Runtime& rt, hipError_t e, const char* what);
 
 // Run body(begin, end) over [0, n) on up to 16 host threads.

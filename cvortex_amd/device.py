@@ -88,6 +88,17 @@ def p3d_eval(prob, stream=None, shape=None):
         raise CvtxError(f"p3d_eval: {last_error()}")
 
 
+def p3d_eval_noalloc(prob, shape, stream=None):
+    """cvtx_hip_p3d_eval_shape_noalloc: the explicit-shape evaluation with
+    its partial sums in the library's fixed scratch array (no allocation)."""
+    st = lib().cvtx_hip_p3d_eval_shape_noalloc(ctypes.byref(prob), shape[0], shape[1], stream)
+    if st != 0:
+        raise CvtxError(f"p3d_eval_noalloc: {last_error()}")
+
+
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
+
 def autotune_p3d(prob, reps=2):
     """Time the candidate launch shapes of a device problem and record the
     fastest for its size bucket (cvtx_hip_autotune_p3d).  Returns

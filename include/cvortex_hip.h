@@ -261,6 +261,22 @@ int cvtx_hip_relax_eval(const float* pv, float* a, int n, float fdt, int regular
  * n_split >= 1 source partitions (rounded to whole 256-source tiles). */
 int cvtx_hip_p3d_eval_shape(const cvtx_hip_p3d_problem* p, int targets_per_lane, int n_split,
                             void* stream);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the same
+ * evaluation as cvtx_hip_p3d_eval_shape, with the FP64 partial sums of a
+ * source-partitioned launch in a fixed device array of
+ * CVTX_HIP_NOALLOC_SCRATCH_BYTES per GPU (held from library load to process
+ * exit) instead of the library's workspace.  Calls no hipMalloc, hipFree or
+ * allocating library routine; targets are not reordered (same results).
+ * Returns -1 without launching when the shape needs more than the array:
+ * n_split x accumulators x n_tgt x 8 bytes, accumulators 3 (vel), 6 (dvort),
+ * 4 (visc), added over the requested outputs.  All calls on one GPU share
+ * its array: calls outside graph capture are ordered across streams, while
+ * concurrent replays of captured calls race. */
+#define CVTX_HIP_NOALLOC_SCRATCH_BYTES (64u << 20)
+int cvtx_hip_p3d_eval_shape_noalloc(const cvtx_hip_p3d_problem* p, int targets_per_lane,
+                                    int n_split, void* stream);
 int cvtx_hip_p2d_eval_shape(const cvtx_hip_p2d_problem* p, int targets_per_lane, int n_split,
                             void* stream);
 int cvtx_hip_f3d_eval_shape(const cvtx_hip_f3d_problem* p, int targets_per_lane, int n_split,
