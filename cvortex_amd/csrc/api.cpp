@@ -297,6 +297,10 @@ int f3d_host(const cvtx_F3D** src, int nf, const bsv_V3f* mes, const cvtx_P3D** 
     j.partial = (double*)device_buffer(rt, *d, SLOT_WORK, wb);
     if (!j.partial) return -1;
   }
+  if (rt.order_targets && f3d_wants_order(nf, nt)) {
+    j.tgt_perm = order_points(rt, *d, d_tpv, nt, SLOT_PERM, d->stream);
+    if (!j.tgt_perm) return -1;
+  }
   if (!check_hip(rt, launch_f3d(j, c, d->stream), "F3D kernel launch")) return -1;
   for (int o = 0; o < 2; ++o)
     if (d_out[o] && !d2h(rt, *d, h_out[o], d_out[o], ob)) return -1;
@@ -696,6 +700,10 @@ static int f3d_dev(const cvtx_hip_f3d_problem* p, const LaunchCfg* forced, void*
   if (wb) {
     j.partial = (double*)device_buffer(rt, *d, SLOT_WORK, wb);
     if (!j.partial) return -1;
+  }
+  if (rt.order_targets && f3d_wants_order(j.n_fil, j.n_tgt)) {
+    j.tgt_perm = order_points(rt, *d, j.tgt_pv, j.n_tgt, SLOT_PERM, pick_stream(d, stream));
+    if (!j.tgt_perm) return -1;
   }
   return check_hip(rt, launch_f3d(j, c, pick_stream(d, stream)), "F3D launch") ? 0 : -1;
 }

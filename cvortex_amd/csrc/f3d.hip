@@ -27,6 +27,7 @@
 #include <cmath>
 
 #include "kernels.hpp"
+#include "tile_cull.hpp"
 
 namespace cvtx {
 
@@ -40,6 +41,7 @@ struct F3DKParams {
   float* __restrict__ out_vel;
   float* __restrict__ out_dvort;
   double* __restrict__ partial;
+  const int* __restrict__ tgt_perm;  // slot -> target (nullptr: identity)
   int n_fil, n_tgt, split_len, n_split;
 };
 
@@ -60,13 +62,94 @@ __device__ __forceinline__ void f3d_finish(const double* a, float3 at, const F3D
   }
 }
 
+// One pair by the general formula (any target position): accumulates into
+// acc[0..NA) of one target.  sa = {a, Gamma/(4pi)}, sb = {b, 3 Gamma/(4pi|r0|)},
+// f0 = {r0}.
+template <int MASK>
+__device__ __forceinline__ void f3d_pair(const float4 fa, const float4 fb, const float4 f0,
+                                         const float3 xt, const float fin, float* acc) {
+  constexpr int D = (MASK & OUT_VEL) ? 3 : 0;
+  const float r1x = xt.x - fa.x, r1y = xt.y - fa.y, r1z = xt.z - fa.z;
+  const float r2x = xt.x - fb.x, r2y = xt.y - fb.y, r2z = xt.z - fb.z;
+  // c = r1 x r2 (= r0 x r1); for targets beside the segment its
+  // products are smaller than those of r0 x r1, so it loses less to
+  // cancellation near the filament
+  const float cx = __builtin_fmaf(r1y, r2z, -r1z * r2y);
+  const float cy = __builtin_fmaf(r1z, r2x, -r1x * r2z);
+  const float cz = __builtin_fmaf(r1x, r2y, -r1y * r2x);
+  const float cc = __builtin_fmaf(cx, cx, __builtin_fmaf(cy, cy, cz * cz));
+  const float n1s = __builtin_fmaf(r1x, r1x, __builtin_fmaf(r1y, r1y, r1z * r1z));
+  const float n2s = __builtin_fmaf(r2x, r2x, __builtin_fmaf(r2y, r2y, r2z * r2z));
+  const float n1 = n1s * __builtin_amdgcn_rsqf(n1s);
+  const float n2 = n2s * __builtin_amdgcn_rsqf(n2s);
+  const float P = n1 * n2;
+  const float sum = n1 + n2;
+  const float d12 = __builtin_fmaf(r1x, r2x, __builtin_fmaf(r1y, r2y, r1z * r2z));
+  const bool same_side = d12 >= 0.f;
+  const float den = same_side ? P * (P + d12) : P * cc;
+  const float num = same_side ? sum : sum * (P - d12);
+  float kk = fa.w * num * __builtin_amdgcn_rcpf(den);
+  if constexpr ((MASK & OUT_DVORT) != 0) {
+    const float rcc = __builtin_amdgcn_rsqf(cc);
+    const float cn = cc * rcc;  // |c|; 0 * inf = NaN when c = 0
+    const float p12 = __builtin_fmaf(f0.x, r1x + r2x, __builtin_fmaf(f0.y, r1y + r2y, f0.z * (r1z + r2z)));
+    float B = -fb.w * cn * p12 * __builtin_amdgcn_rcpf(P * sum);
+    // B is NaN whenever c = 0 (target on the filament's line), so the
+    // finiteness of B also drops those pairs
+    const bool ok = (__builtin_fabsf(kk) < __builtin_inff()) &&
+                    (__builtin_fabsf(B) < __builtin_inff());
+    B = ok ? B : 0.f;
+    const float kd = ok ? kk : 0.f;
+    if constexpr ((MASK & OUT_VEL) != 0) {
+      kk = (__builtin_fabsf(kk) < __builtin_inff()) ? kk : 0.f;
+      acc[0] = __builtin_fmaf(kk, cx, acc[0]);
+      acc[1] = __builtin_fmaf(kk, cy, acc[1]);
+      acc[2] = __builtin_fmaf(kk, cz, acc[2]);
+    }
+    acc[D + 0] += B;
+    acc[D + 1] = __builtin_fmaf(kd, f0.x, acc[D + 1]);
+    acc[D + 2] = __builtin_fmaf(kd, f0.y, acc[D + 2]);
+    acc[D + 3] = __builtin_fmaf(kd, f0.z, acc[D + 3]);
+  } else {
+    kk = (__builtin_fabsf(kk) < fin) ? kk : 0.f;
+    acc[0] = __builtin_fmaf(kk, cx, acc[0]);
+    acc[1] = __builtin_fmaf(kk, cy, acc[1]);
+    acc[2] = __builtin_fmaf(kk, cz, acc[2]);
+  }
+}
+
+// Far tiles (tile_cull.hpp).  When every target of the block is at least
+// half the longest filament of the tile away from the tile's box, every
+// pair has r1.r2 = |x - m|^2 - |r0|^2/4 >= 0 (m the filament's midpoint),
+// so the same-side form holds for all of them, with y_i = 1/|r_i|:
+//   k = (Gamma/4pi) (|r1| + |r2|) / (|r1||r2| (|r1||r2| + r1.r2))
+//     = (Gamma/4pi) (y1 + y2) / (P + r1.r2),           P = |r1||r2|,
+//   1/|r1| - 1/|r2| = -r0.(r1 + r2) y1 y2 / (|r1| + |r2|),
+// and no pair's terms can be infinite.  The reciprocals of P + r1.r2 (and,
+// for the stretching, of |r1| + |r2|) of the group's pairs come from one
+// shared reciprocal (batch_rcp).  The stretching terms of a pair with c = 0
+// (a target on the extension of a filament's line) are dropped, as the
+// general form does (there |c| = 0 and the reference's terms are NaN).
+template <int MASK, int T>
+struct F3DFar {
+  // pairs per group and reciprocals per pair
+  static constexpr int G = (MASK == OUT_VEL) ? 4 : 2;
+  static constexpr int U = G >= T ? G / T : 1;  // filaments per group
+  static constexpr int NP = U * T;              // pairs per group
+  static constexpr int NR = (MASK & OUT_DVORT) ? 2 : 1;
+  static constexpr int N = NP * NR;             // values per shared reciprocal
+};
+
 template <int MASK, int ACC, int T, int WG>
 __global__ __launch_bounds__(WG) void f3d_kernel(F3DKParams p) {
   constexpr int NA = ((MASK & OUT_VEL) ? 3 : 0) + ((MASK & OUT_DVORT) ? 4 : 0);
   constexpr int D = (MASK & OUT_VEL) ? 3 : 0;
+  constexpr int WAVES = WG / 64;
+  using FF = F3DFar<MASK, T>;
   __shared__ float4 s_a[WG];
   __shared__ float4 s_b[WG];
   __shared__ float4 s_r0[WG];
+  __shared__ BoxPartials<3, WAVES> s_box;
   const int tid = threadIdx.x;
   const int base = blockIdx.x * (WG * T);
   float3 xt[T], at[T];
@@ -74,6 +157,7 @@ __global__ __launch_bounds__(WG) void f3d_kernel(F3DKParams p) {
   for (int k = 0; k < T; ++k) {
     int i = base + k * WG + tid;
     i = i < p.n_tgt ? i : p.n_tgt - 1;
+    if (p.tgt_perm) i = p.tgt_perm[i];
     const float4 q = p.tgt_pv[i];
     xt[k] = make_float3(q.x, q.y, q.z);
     if constexpr ((MASK & OUT_DVORT) != 0) {
@@ -89,21 +173,49 @@ __global__ __launch_bounds__(WG) void f3d_kernel(F3DKParams p) {
 #pragma unroll
     for (int a = 0; a < NA; ++a) { acc[k][a] = 0.f; hi[k][a] = 0.f; lo[k][a] = 0.f; }
 
+  // box of the block's targets (the clamped duplicates are real targets)
+  Box<3> tbox;
+  {
+    float l3[3] = {xt[0].x, xt[0].y, xt[0].z}, h3[3] = {xt[0].x, xt[0].y, xt[0].z};
+    bool fin = true;
+#pragma unroll
+    for (int k = 0; k < T; ++k) {
+      l3[0] = fminf(l3[0], xt[k].x); l3[1] = fminf(l3[1], xt[k].y); l3[2] = fminf(l3[2], xt[k].z);
+      h3[0] = fmaxf(h3[0], xt[k].x); h3[1] = fmaxf(h3[1], xt[k].y); h3[2] = fmaxf(h3[2], xt[k].z);
+      fin = fin && __builtin_isfinite(xt[k].x) && __builtin_isfinite(xt[k].y) &&
+            __builtin_isfinite(xt[k].z);
+    }
+    tbox = block_box<3, WAVES>(l3, h3, fin, s_box);
+  }
+
   const float k4pi = (float)(1.0 / (4.0 * M_PI));
   const int s0 = blockIdx.y * p.split_len;
   const int s1 = min(s0 + p.split_len, p.n_fil);
   for (int ts = s0; ts < s1; ts += WG) {
     const int cnt = min(WG, s1 - ts);
     __syncthreads();
-    if (tid < cnt) {
-      const float4 a = p.fa[ts + tid];
-      const float4 b = p.fb[ts + tid];
+    {
+      float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+      if (tid < cnt) {
+        a = p.fa[ts + tid];
+        b = p.fb[ts + tid];
+        const float rx = b.x - a.x, ry = b.y - a.y, rz = b.z - a.z;
+        const float g = a.w * k4pi;
+        const float len = __builtin_sqrtf(rx * rx + ry * ry + rz * rz);
+        s_a[tid] = make_float4(a.x, a.y, a.z, g);
+        s_b[tid] = make_float4(b.x, b.y, b.z, 3.f * g / len);
+        s_r0[tid] = make_float4(rx, ry, rz, 0.f);
+      }
+      // the tile's box holds both ends of every filament; extra: the
+      // longest filament's squared length
+      const float l3[3] = {fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)};
+      const float h3[3] = {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)};
       const float rx = b.x - a.x, ry = b.y - a.y, rz = b.z - a.z;
-      const float g = a.w * k4pi;
-      const float len = __builtin_sqrtf(rx * rx + ry * ry + rz * rz);
-      s_a[tid] = make_float4(a.x, a.y, a.z, g);
-      s_b[tid] = make_float4(b.x, b.y, b.z, 3.f * g / len);
-      s_r0[tid] = make_float4(rx, ry, rz, 0.f);
+      const bool fin = __builtin_isfinite(a.x) && __builtin_isfinite(a.y) &&
+                       __builtin_isfinite(a.z) && __builtin_isfinite(b.x) &&
+                       __builtin_isfinite(b.y) && __builtin_isfinite(b.z) &&
+                       __builtin_isfinite(a.w);
+      box_partial<3, WAVES>(l3, h3, rx * rx + ry * ry + rz * rz, fin, tid < cnt, s_box);
     }
     __syncthreads();
     if constexpr (ACC == ACC_COMP) {
@@ -122,60 +234,94 @@ __global__ __launch_bounds__(WG) void f3d_kernel(F3DKParams p) {
       if constexpr ((MASK & OUT_DVORT) == 0) return in_vgpr(__builtin_inff());
       else return __builtin_inff();
     }();
-#pragma unroll 2
-    for (int j = 0; j < cnt; ++j, ++jv) {
-      const float4 fa = s_a[jv];
-      const float4 fb = s_b[jv];
-      const float4 f0 = s_r0[jv];
+    float l2max;
+    const Box<3> fbox = box_combine<3, WAVES>(s_box, &l2max);
+    // every pair same-side and finite, and each of the N values that share
+    // a reciprocal (P + r1.r2 in [dmin^2, 2 dmax^2], |r1| + |r2| in
+    // [2 dmin, 2 dmax]) within 2^(+-120/N), so their products stay normal
+    bool far = false;
+    if (tbox.finite && fbox.finite) {
+      float dmin2, dmax2;
+      box_dist2<3>(tbox, fbox, dmin2, dmax2);
+      const float lo_ok = FF::N <= 4 ? 9.3e-10f : 3.1e-5f;   // 2^-30, 2^-15
+      const float hi_ok = FF::N <= 4 ? 5.3e8f : 1.6e4f;      // 2^29, 2^14
+      far = dmin2 >= 0.2501f * l2max && dmin2 >= lo_ok && dmax2 <= hi_ok;
+    }
+    if (far) {
+      constexpr int U = FF::U;
+      int j = 0;
+      for (; j + U <= cnt; j += U, jv += U) {
+        float4 fa[U], fb[U], f0[U];
 #pragma unroll
-      for (int k = 0; k < T; ++k) {
-        const float r1x = xt[k].x - fa.x, r1y = xt[k].y - fa.y, r1z = xt[k].z - fa.z;
-        const float r2x = xt[k].x - fb.x, r2y = xt[k].y - fb.y, r2z = xt[k].z - fb.z;
-        // c = r1 x r2 (= r0 x r1); for targets beside the segment its
-        // products are smaller than those of r0 x r1, so it loses less to
-        // cancellation near the filament
-        const float cx = __builtin_fmaf(r1y, r2z, -r1z * r2y);
-        const float cy = __builtin_fmaf(r1z, r2x, -r1x * r2z);
-        const float cz = __builtin_fmaf(r1x, r2y, -r1y * r2x);
-        const float cc = __builtin_fmaf(cx, cx, __builtin_fmaf(cy, cy, cz * cz));
-        const float n1s = __builtin_fmaf(r1x, r1x, __builtin_fmaf(r1y, r1y, r1z * r1z));
-        const float n2s = __builtin_fmaf(r2x, r2x, __builtin_fmaf(r2y, r2y, r2z * r2z));
-        const float n1 = n1s * __builtin_amdgcn_rsqf(n1s);
-        const float n2 = n2s * __builtin_amdgcn_rsqf(n2s);
-        const float P = n1 * n2;
-        const float sum = n1 + n2;
-        const float d12 = __builtin_fmaf(r1x, r2x, __builtin_fmaf(r1y, r2y, r1z * r2z));
-        const bool same_side = d12 >= 0.f;
-        const float den = same_side ? P * (P + d12) : P * cc;
-        const float num = same_side ? sum : sum * (P - d12);
-        float kk = fa.w * num * __builtin_amdgcn_rcpf(den);
-        if constexpr ((MASK & OUT_DVORT) != 0) {
-          const float rcc = __builtin_amdgcn_rsqf(cc);
-          const float cn = cc * rcc;  // |c|; 0 * inf = NaN when c = 0
-          const float p12 = __builtin_fmaf(f0.x, r1x + r2x, __builtin_fmaf(f0.y, r1y + r2y, f0.z * (r1z + r2z)));
-          float B = -fb.w * cn * p12 * __builtin_amdgcn_rcpf(P * sum);
-          // B is NaN whenever c = 0 (target on the filament's line), so the
-          // finiteness of B also drops those pairs
-          const bool ok = (__builtin_fabsf(kk) < __builtin_inff()) &&
-                          (__builtin_fabsf(B) < __builtin_inff());
-          B = ok ? B : 0.f;
-          const float kd = ok ? kk : 0.f;
-          if constexpr ((MASK & OUT_VEL) != 0) {
-            kk = (__builtin_fabsf(kk) < __builtin_inff()) ? kk : 0.f;
-            acc[k][0] = __builtin_fmaf(kk, cx, acc[k][0]);
-            acc[k][1] = __builtin_fmaf(kk, cy, acc[k][1]);
-            acc[k][2] = __builtin_fmaf(kk, cz, acc[k][2]);
-          }
-          acc[k][D + 0] += B;
-          acc[k][D + 1] = __builtin_fmaf(kd, f0.x, acc[k][D + 1]);
-          acc[k][D + 2] = __builtin_fmaf(kd, f0.y, acc[k][D + 2]);
-          acc[k][D + 3] = __builtin_fmaf(kd, f0.z, acc[k][D + 3]);
-        } else {
-          kk = (__builtin_fabsf(kk) < fin) ? kk : 0.f;
-          acc[k][0] = __builtin_fmaf(kk, cx, acc[k][0]);
-          acc[k][1] = __builtin_fmaf(kk, cy, acc[k][1]);
-          acc[k][2] = __builtin_fmaf(kk, cz, acc[k][2]);
+        for (int u = 0; u < U; ++u) {
+          fa[u] = s_a[jv + u];
+          fb[u] = s_b[jv + u];
+          f0[u] = s_r0[jv + u];
         }
+        float cx[FF::NP], cy[FF::NP], cz[FF::NP], ys[FF::NP], y12[FF::NP], p12[FF::NP],
+            cn[FF::NP], den[FF::N], inv[FF::N];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int k = 0; k < T; ++k) {
+            const int q = u * T + k;
+            const float r1x = xt[k].x - fa[u].x, r1y = xt[k].y - fa[u].y, r1z = xt[k].z - fa[u].z;
+            const float r2x = xt[k].x - fb[u].x, r2y = xt[k].y - fb[u].y, r2z = xt[k].z - fb[u].z;
+            cx[q] = __builtin_fmaf(r1y, r2z, -r1z * r2y);
+            cy[q] = __builtin_fmaf(r1z, r2x, -r1x * r2z);
+            cz[q] = __builtin_fmaf(r1x, r2y, -r1y * r2x);
+            const float n1s = __builtin_fmaf(r1x, r1x, __builtin_fmaf(r1y, r1y, r1z * r1z));
+            const float n2s = __builtin_fmaf(r2x, r2x, __builtin_fmaf(r2y, r2y, r2z * r2z));
+            const float y1 = __builtin_amdgcn_rsqf(n1s);
+            const float y2 = __builtin_amdgcn_rsqf(n2s);
+            const float n1 = n1s * y1, n2 = n2s * y2;
+            const float d12 = __builtin_fmaf(r1x, r2x, __builtin_fmaf(r1y, r2y, r1z * r2z));
+            ys[q] = y1 + y2;
+            den[q] = __builtin_fmaf(n1, n2, d12);  // P + r1.r2
+            if constexpr ((MASK & OUT_DVORT) != 0) {
+              y12[q] = y1 * y2;
+              den[FF::NP + q] = n1 + n2;
+              const float cc = __builtin_fmaf(cx[q], cx[q], __builtin_fmaf(cy[q], cy[q], cz[q] * cz[q]));
+              cn[q] = __builtin_amdgcn_sqrtf(cc);
+              p12[q] = __builtin_fmaf(f0[u].x, r1x + r2x,
+                                      __builtin_fmaf(f0[u].y, r1y + r2y, f0[u].z * (r1z + r2z)));
+            }
+          }
+        batch_rcp<FF::N>(den, inv);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int k = 0; k < T; ++k) {
+            const int q = u * T + k;
+            const float kk = fa[u].w * ys[q] * inv[q];
+            if constexpr ((MASK & OUT_VEL) != 0) {
+              acc[k][0] = __builtin_fmaf(kk, cx[q], acc[k][0]);
+              acc[k][1] = __builtin_fmaf(kk, cy[q], acc[k][1]);
+              acc[k][2] = __builtin_fmaf(kk, cz[q], acc[k][2]);
+            }
+            if constexpr ((MASK & OUT_DVORT) != 0) {
+              const float B = -fb[u].w * cn[q] * p12[q] * y12[q] * inv[FF::NP + q];
+              const float kd = cn[q] > 0.f ? kk : 0.f;
+              acc[k][D + 0] += B;
+              acc[k][D + 1] = __builtin_fmaf(kd, f0[u].x, acc[k][D + 1]);
+              acc[k][D + 2] = __builtin_fmaf(kd, f0[u].y, acc[k][D + 2]);
+              acc[k][D + 3] = __builtin_fmaf(kd, f0[u].z, acc[k][D + 3]);
+            }
+          }
+      }
+      for (; j < cnt; ++j, ++jv) {
+        const float4 fa = s_a[jv], fb = s_b[jv], f0 = s_r0[jv];
+#pragma unroll
+        for (int k = 0; k < T; ++k) f3d_pair<MASK>(fa, fb, f0, xt[k], fin, acc[k]);
+      }
+    } else {
+#pragma unroll 2
+      for (int j = 0; j < cnt; ++j, ++jv) {
+        const float4 fa = s_a[jv];
+        const float4 fb = s_b[jv];
+        const float4 f0 = s_r0[jv];
+#pragma unroll
+        for (int k = 0; k < T; ++k) f3d_pair<MASK>(fa, fb, f0, xt[k], fin, acc[k]);
       }
     }
     if constexpr (ACC == ACC_COMP) {
@@ -195,7 +341,7 @@ __global__ __launch_bounds__(WG) void f3d_kernel(F3DKParams p) {
     for (int a = 0; a < NA; ++a)
       d[a] = (ACC == ACC_COMP) ? (double)hi[k][a] + (double)lo[k][a] : (double)acc[k][a];
     if (p.n_split == 1) {
-      f3d_finish<MASK>(d, at[k], p, i);
+      f3d_finish<MASK>(d, at[k], p, p.tgt_perm ? p.tgt_perm[i] : i);
     } else {
 #pragma unroll
       for (int a = 0; a < NA; ++a) p.partial[((size_t)blockIdx.y * NA + a) * p.n_tgt + i] = d[a];
@@ -214,12 +360,13 @@ __global__ __launch_bounds__(256) void f3d_reduce(F3DKParams p) {
   for (int s = 0; s < p.n_split; ++s)
 #pragma unroll
     for (int a = 0; a < NA; ++a) d[a] += p.partial[((size_t)s * NA + a) * p.n_tgt + i];
+  const int t = p.tgt_perm ? p.tgt_perm[i] : i;
   float3 at = make_float3(0.f, 0.f, 0.f);
   if constexpr ((MASK & OUT_DVORT) != 0) {
-    const float4 w = p.tgt_a[i];
+    const float4 w = p.tgt_a[t];
     at = make_float3(w.x, w.y, w.z);
   }
-  f3d_finish<MASK>(d, at, p, i);
+  f3d_finish<MASK>(d, at, p, t);
 }
 
 size_t f3d_workspace_bytes(const F3DJob& j, const LaunchCfg& c) {
@@ -335,6 +482,7 @@ hipError_t launch_f3d(const F3DJob& j, const LaunchCfg& c, hipStream_t s) {
   p.out_vel = j.out_vel;
   p.out_dvort = j.out_dvort;
   p.partial = j.partial;
+  p.tgt_perm = j.tgt_perm;
   p.n_fil = j.n_fil;
   p.n_tgt = j.n_tgt;
   int n_split = c.n_split, split_len = c.split_len;

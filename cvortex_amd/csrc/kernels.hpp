@@ -76,6 +76,7 @@ struct F3DJob {
   int mask = OUT_VEL;             // OUT_VEL and/or OUT_DVORT
   int acc = ACC_COMP;
   double* partial = nullptr;
+  const int* tgt_perm = nullptr;  // spatial order of the targets (order.hip), or identity
 };
 
 // Vorticity field (cvtx_P3D_M2M_vort semantics: 5 sigma box cut-off).
@@ -169,6 +170,12 @@ inline bool vort_wants_sort(int n_src, int n_tgt) {
 
 inline bool p3d_wants_order(int reg, int n_src, int n_tgt) {
   return reg == REG_GAUSSIAN && n_tgt >= 4096 && (long long)n_src * n_tgt >= (1LL << 26);
+}
+
+// Filament kernels: spatially ordered targets make a block's box small, so
+// most filament tiles are far from it and take the far-tile loop (f3d.hip).
+inline bool f3d_wants_order(int n_fil, int n_tgt) {
+  return n_tgt >= 4096 && (long long)n_fil * n_tgt >= (1LL << 26);
 }
 
 }  // namespace cvtx

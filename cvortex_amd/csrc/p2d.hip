@@ -15,6 +15,7 @@
 #include <cmath>
 
 #include "kernels.hpp"
+#include "tile_cull.hpp"
 
 namespace cvtx {
 
@@ -127,11 +128,32 @@ __device__ __forceinline__ void p2d_finish(const double* a, float4 t, const P2DK
   }
 }
 
+// Far tiles (tile_cull.hpp): for planetary and Gaussian, a pair at
+// r^2 >= near2 has K = 1/r^2 in the kernel's units (planetary: r^2 >=
+// sigma^2, outside the core; Gaussian: rho^2 >= 208, where exp(-rho^2/2) is
+// 0 in FP32 and K = 1/rho^2 = sigma^2/r^2, eta = 0), so a tile whose box is
+// at least that far from the block's targets needs no range test: its
+// pairs take K from one shared reciprocal per four pairs.
+template <int REG>
+struct P2DFar {
+  static constexpr bool ENABLED = REG == REG_PLANETARY || REG == REG_GAUSSIAN;
+  static __device__ __forceinline__ float near2(const RegConsts& k) {
+    return REG == REG_PLANETARY ? k.s2 : k.q_far * k.s2;
+  }
+  // factor of 1/r^2 in K
+  static __device__ __forceinline__ float kscale(const RegConsts& k) {
+    return REG == REG_PLANETARY ? 1.f : k.s2;
+  }
+};
+
 template <int REG, int MASK, int ACC, int T, int WG>
 __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
   constexpr int NA = ((MASK & OUT_VEL) ? 2 : 0) + ((MASK & OUT_VISC) ? 2 : 0);
   constexpr int E = (MASK & OUT_VEL) ? 2 : 0;
+  constexpr bool CULL = P2DFar<REG>::ENABLED && (MASK & OUT_VEL) != 0;
+  constexpr int WAVES = WG / 64;
   __shared__ float4 s_src[WG];
+  __shared__ BoxPartials<2, WAVES> s_box;
   const int tid = threadIdx.x;
   const int base = blockIdx.x * (WG * T);
   float4 xt[T];
@@ -147,13 +169,36 @@ __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
 #pragma unroll
     for (int a = 0; a < NA; ++a) { acc[k][a] = 0.f; hi[k][a] = 0.f; lo[k][a] = 0.f; }
 
+  // box of the block's targets (the clamped duplicates are real targets)
+  Box<2> tbox;
+  if constexpr (CULL) {
+    float lo2[2] = {xt[0].x, xt[0].y}, hi2[2] = {xt[0].x, xt[0].y};
+#pragma unroll
+    for (int k = 1; k < T; ++k) {
+      lo2[0] = fminf(lo2[0], xt[k].x); lo2[1] = fminf(lo2[1], xt[k].y);
+      hi2[0] = fmaxf(hi2[0], xt[k].x); hi2[1] = fmaxf(hi2[1], xt[k].y);
+    }
+    bool fin = true;
+#pragma unroll
+    for (int k = 0; k < T; ++k) fin = fin && __builtin_isfinite(xt[k].x) && __builtin_isfinite(xt[k].y);
+    tbox = block_box<2, WAVES>(lo2, hi2, fin, s_box);
+  }
+
   const int s0 = blockIdx.y * p.split_len;
   const int s1 = min(s0 + p.split_len, p.n_src);
   const RegConsts rv = loop_consts(p.rc);
   for (int ts = s0; ts < s1; ts += WG) {
     const int cnt = min(WG, s1 - ts);
     __syncthreads();
-    if (tid < cnt) s_src[tid] = p.src[ts + tid];
+    float4 src = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (tid < cnt) {
+      src = p.src[ts + tid];
+      s_src[tid] = src;
+    }
+    if constexpr (CULL) {
+      const float x2[2] = {src.x, src.y};
+      box_partial<2, WAVES>(x2, tid < cnt, s_box);
+    }
     __syncthreads();
     if constexpr (ACC == ACC_COMP) {
 #pragma unroll
@@ -162,26 +207,80 @@ __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
         for (int a = 0; a < NA; ++a) acc[k][a] = 0.f;
     }
     int jv = in_vgpr(0);  // LDS index in a VGPR (no SGPR operand per source)
-    // (the Gaussian's pinned branch cannot be unrolled around)
-    constexpr int kUnroll = REG == REG_GAUSSIAN ? 1 : 4;
-#pragma unroll kUnroll
-    for (int j = 0; j < cnt; ++j, ++jv) {
-      const float4 s = s_src[jv];
+    bool far = false;
+    if constexpr (CULL) far = boxes_far<2>(tbox, box_combine<2, WAVES>(s_box), P2DFar<REG>::near2(p.rc));
+    if (far) {
+      if constexpr (CULL) {
+        // U sources x T targets = 4 pairs per shared reciprocal; eta = 0
+        constexpr int U = 4 / T;
+        const float ks = in_vgpr(P2DFar<REG>::kscale(p.rc));
+        int j = 0;
+        for (; j + U <= cnt; j += U, jv += U) {
+          float4 s[U];
 #pragma unroll
-      for (int k = 0; k < T; ++k) {
-        const float dx = xt[k].x - s.x;
-        const float dy = xt[k].y - s.y;
-        const float r2 = __builtin_fmaf(dx, dx, dy * dy);
-        float K, eta;
-        Reg2D<REG>::eval(r2, rv, K, eta);
-        if constexpr ((MASK & OUT_VEL) != 0) {
-          const float ka = K * s.z;
-          acc[k][0] = __builtin_fmaf(ka, dy, acc[k][0]);
-          acc[k][1] = __builtin_fmaf(ka, dx, acc[k][1]);
+          for (int u = 0; u < U; ++u) s[u] = s_src[jv + u];
+          float dx[4], dy[4], r2[4], K[4];
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < T; ++k) {
+              const int q = u * T + k;
+              dx[q] = xt[k].x - s[u].x;
+              dy[q] = xt[k].y - s[u].y;
+              r2[q] = __builtin_fmaf(dx[q], dx[q], dy[q] * dy[q]);
+            }
+          // K[q] a_s for the four pairs
+          if constexpr (T == 1) {
+            shared_rcp4(r2, K);
+#pragma unroll
+            for (int u = 0; u < U; ++u) K[u] *= s[u].z * ks;
+          } else {
+            // pairs 0, 1 and 2, 3 share a source: its strength scales the
+            // shared reciprocal once
+            shared_rcp4_scaled(r2, s[0].z * ks, s[U - 1].z * ks, K);
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int k = q % T;
+            acc[k][0] = __builtin_fmaf(K[q], dy[q], acc[k][0]);
+            acc[k][1] = __builtin_fmaf(K[q], dx[q], acc[k][1]);
+          }
         }
-        if constexpr ((MASK & OUT_VISC) != 0) {
-          acc[k][E + 0] = __builtin_fmaf(eta, s.z, acc[k][E + 0]);
-          acc[k][E + 1] = __builtin_fmaf(eta, s.w, acc[k][E + 1]);
+        for (; j < cnt; ++j, ++jv) {
+          const float4 s = s_src[jv];
+          const float as = s.z * ks;
+#pragma unroll
+          for (int k = 0; k < T; ++k) {
+            const float dx = xt[k].x - s.x;
+            const float dy = xt[k].y - s.y;
+            const float ka = __builtin_amdgcn_rcpf(__builtin_fmaf(dx, dx, dy * dy)) * as;
+            acc[k][0] = __builtin_fmaf(ka, dy, acc[k][0]);
+            acc[k][1] = __builtin_fmaf(ka, dx, acc[k][1]);
+          }
+        }
+      }
+    } else {
+      // (the Gaussian's pinned branch cannot be unrolled around)
+      constexpr int kUnroll = REG == REG_GAUSSIAN ? 1 : 4;
+#pragma unroll kUnroll
+      for (int j = 0; j < cnt; ++j, ++jv) {
+        const float4 s = s_src[jv];
+#pragma unroll
+        for (int k = 0; k < T; ++k) {
+          const float dx = xt[k].x - s.x;
+          const float dy = xt[k].y - s.y;
+          const float r2 = __builtin_fmaf(dx, dx, dy * dy);
+          float K, eta;
+          Reg2D<REG>::eval(r2, rv, K, eta);
+          if constexpr ((MASK & OUT_VEL) != 0) {
+            const float ka = K * s.z;
+            acc[k][0] = __builtin_fmaf(ka, dy, acc[k][0]);
+            acc[k][1] = __builtin_fmaf(ka, dx, acc[k][1]);
+          }
+          if constexpr ((MASK & OUT_VISC) != 0) {
+            acc[k][E + 0] = __builtin_fmaf(eta, s.z, acc[k][E + 0]);
+            acc[k][E + 1] = __builtin_fmaf(eta, s.w, acc[k][E + 1]);
+          }
         }
       }
     }

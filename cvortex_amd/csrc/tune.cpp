@@ -94,14 +94,18 @@ LaunchCfg choose_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt,
   }
   LaunchCfg c = default_cfg(n_src, n_tgt, n_cu);
   const bool many_blocks = (long long)(n_tgt + 255) / 256 >= 4LL * cus;
-  // Filaments with one output (vel or dvort alone): one target per lane fits
-  // in 48-50 VGPRs, eight waves per SIMD against five or six with two, and
-  // the extra waves pair more instructions for dual issue than the second
-  // target's reuse saves (16384 filaments x 2^20 targets, no partitions: vel
-  // 16.9 ms against 17.6, dvort 23.1 against 24.4; with 4 partitions vel
-  // 16.1 against 17.6; the fused kernel keeps two, 24.4 against 25.1).
-  if (kind == KIND_F3D && (mask == OUT_VEL || mask == OUT_DVORT) && c.T == 2 && many_blocks)
-    c = partitioned_cfg(1, n_src, n_tgt, n_cu);
+  // Filaments: one target per lane.  It fits in fewer VGPRs (more waves per
+  // SIMD to pair instructions for dual issue) and, with ordered targets,
+  // makes a block's box smaller, so more filament tiles take the far-tile
+  // loop.  16384 filaments x 2^20 targets, 4 partitions (ms, two targets
+  // per lane -> one): vel 13.7 -> 13.1, dvort 21.1 -> 19.2, fused 21.4 ->
+  // 20.4.
+  if (kind == KIND_F3D && c.T > 1 && many_blocks) c = partitioned_cfg(1, n_src, n_tgt, n_cu);
+  // 2D Gaussian with the viscous term: two targets per lane (2^22 sheet,
+  // 4 partitions: 3331 ms with four, 3226 with two); planetary and the
+  // velocity alone keep four (2720 ms against 3042 with two).
+  if (kind == KIND_P2D && reg == REG_GAUSSIAN && (mask & OUT_VISC) && c.T == 4)
+    c = partitioned_cfg(2, n_src, n_tgt, n_cu);
   return c;
 }
 

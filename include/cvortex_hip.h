@@ -163,9 +163,11 @@ int cvtx_hip_host_threads(void);
  * previous mode, or -1 for an invalid mode. Default CVTX_HIP_ACC_COMP. */
 int cvtx_hip_set_accumulation(int mode);
 int cvtx_hip_get_accumulation(void);
-/* Spatial (Morton) ordering of targets before Gaussian P3D launches (on by
- * default): waves of nearby targets skip the near-field table for distant
- * sources.  Results do not depend on it.  Returns the previous setting. */
+/* Spatial (Morton) ordering of targets (on by default) before Gaussian P3D
+ * launches, where waves of nearby targets skip the near-field table for
+ * distant sources, and before filament (F3D) launches, where compact target
+ * blocks let most filament tiles take the far-tile loop.  Results agree to
+ * rounding either way.  Returns the previous setting. */
 int cvtx_hip_set_target_order(int enable);
 /* The library's stream on the active device (hipStream_t as void*), NULL
  * when no device is enabled. */
