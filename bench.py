@@ -352,16 +352,14 @@ def bench_clouds(args, cv, D, W, o, stream):
     vel, dv = D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
     tm = Timer(stream)
     b = D.Batch(off, sig, "winckelmans", outputs=cv.OUT_VEL | cv.OUT_DVORT, accumulate=cv.ACC_COMP)
-    # autotune the persistent launch shape: targets per lane x workgroups per CU
-    best = None
-    for T in (1, 2, 4):
-        for wpc in (2, 4, 8):
-            b.set_shape(T, wpc)
-            s, k = tm.run(lambda: b.eval(dpv, da, vel, dv, stream=stream), 1, 1)
-            log(f"  shape T={T} wg/cu={wpc}: {k:.1f} ms")
-            if best is None or k < best[0]:
-                best = (k, T, wpc)
-    b.set_shape(best[1], best[2])
+    # the library's autotuner picks the persistent launch shape (targets per
+    # lane x workgroup size x resident workgroups per CU) and caches it
+    shape0 = b.shape()
+    t0 = time.perf_counter()
+    tuned = b.autotune(dpv, da, vel, dv, reps=1)
+    tune_s = time.perf_counter() - t0
+    log(f"  autotuned shape T={tuned[0]} wg={tuned[1]} wg/cu={tuned[2]}: {tuned[3]:.1f} ms "
+        f"(from {shape0}, {tune_s:.1f} s)")
     ms_step, ms_k = tm.run(lambda: b.eval(dpv, da, vel, dv, stream=stream), args.steps, args.warmup)
     V, Dv = vel.download(), dv.download()
     v2, d2 = D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
@@ -420,7 +418,9 @@ def bench_clouds(args, cv, D, W, o, stream):
                      {"workload": "512 clouds, sizes log-uniform in [1024, 65536], unit cubes, "
                                   "Winckelmans sigma_c ~ U[0.01, 0.1], fused vel+dvort",
                       "n_clouds": n_clouds, "n_particles": n, "pairs_per_eval": pairs,
-                      "shape": {"targets_per_lane": best[1], "workgroups_per_cu": best[2]}})
+                      "shape": {"targets_per_lane": tuned[0], "workgroup_size": tuned[1],
+                                "workgroups_per_cu": tuned[2], "tuned_by": "cvtx_hip_batch_autotune",
+                                "tuning_s": tune_s, "shape_before_tuning": list(shape0)}})
     line.update({"roofline": roofline(pairs, F, ms_k),
                  "per_cloud_plain_launches": {
                      "what": "one launch per cloud, 1 target per lane, no source partitions, "

@@ -146,6 +146,9 @@ _SIGNATURES = {
     "cvtx_hip_batch_destroy": (None, [_VP]),
     "cvtx_hip_batch_set_shape": (_I, [_VP, _I, _I]),
     "cvtx_hip_batch_info": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "cvtx_hip_batch_shape": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "cvtx_hip_batch_autotune": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _I, ctypes.POINTER(_I),
+                                     ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_F)]),
     "cvtx_hip_batch_eval": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "cvtx_hip_batch_eval_per_cloud": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _I, _VP]),
     "cvtx_hip_batch_rk4_init": (_I, [_VP, _VP, _VP, _F]),

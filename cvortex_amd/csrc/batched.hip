@@ -33,6 +33,7 @@
 
 #include "p3d_body.hpp"
 #include "runtime.hpp"
+#include "tune.hpp"
 
 namespace cvtx {
 
@@ -118,27 +119,33 @@ namespace {
 
 using BFn = void (*)(BatchParams);
 
-template <int REG, int MASK, int ACC>
+template <int REG, int MASK, int ACC, int WG>
 BFn b_T(int T) {
   switch (T) {
-    case 1: return p3d_batched_kernel<REG, MASK, ACC, 1, 256>;
-    case 4: return p3d_batched_kernel<REG, MASK, ACC, 4, 256>;
-    default: return p3d_batched_kernel<REG, MASK, ACC, 2, 256>;
+    case 1: return p3d_batched_kernel<REG, MASK, ACC, 1, WG>;
+    case 4: return p3d_batched_kernel<REG, MASK, ACC, 4, WG>;
+    default: return p3d_batched_kernel<REG, MASK, ACC, 2, WG>;
   }
 }
+template <int REG, int MASK, int ACC>
+BFn b_wg(int wg, int T) {
+  return wg == 128 ? b_T<REG, MASK, ACC, 128>(T) : b_T<REG, MASK, ACC, 256>(T);
+}
 template <int REG, int MASK>
-BFn b_acc(int acc, int T) {
-  return acc == ACC_PLAIN ? b_T<REG, MASK, ACC_PLAIN>(T) : b_T<REG, MASK, ACC_COMP>(T);
+BFn b_acc(int acc, int wg, int T) {
+  return acc == ACC_PLAIN ? b_wg<REG, MASK, ACC_PLAIN>(wg, T) : b_wg<REG, MASK, ACC_COMP>(wg, T);
 }
 template <int REG>
-BFn b_mask(int mask, int T, int acc) {
-  if (mask == (OUT_VEL | OUT_DVORT)) return b_acc<REG, OUT_VEL | OUT_DVORT>(acc, T);
-  if (mask == (OUT_VEL | OUT_DVORT | OUT_VISC)) return b_acc<REG, OUT_VEL | OUT_DVORT | OUT_VISC>(acc, T);
+BFn b_mask(int mask, int wg, int T, int acc) {
+  if (mask == (OUT_VEL | OUT_DVORT)) return b_acc<REG, OUT_VEL | OUT_DVORT>(acc, wg, T);
+  if (mask == (OUT_VEL | OUT_DVORT | OUT_VISC))
+    return b_acc<REG, OUT_VEL | OUT_DVORT | OUT_VISC>(acc, wg, T);
   return nullptr;
 }
-BFn b_pick(int reg, int mask, int acc, int T) {
-  if (reg == REG_WINCKELMANS) return b_mask<REG_WINCKELMANS>(mask, T, acc);
-  if (reg == REG_GAUSSIAN) return b_mask<REG_GAUSSIAN>(mask, T, acc);
+BFn b_pick(int reg, int mask, int acc, int wg, int T) {
+  if (wg != 128 && wg != 256) return nullptr;
+  if (reg == REG_WINCKELMANS) return b_mask<REG_WINCKELMANS>(mask, wg, T, acc);
+  if (reg == REG_GAUSSIAN) return b_mask<REG_GAUSSIAN>(mask, wg, T, acc);
   return nullptr;
 }
 
@@ -158,6 +165,8 @@ struct cvtx_hip_batch {
   int acc = ACC_COMP;
   float nu = 0.f;
   int T = 2;
+  int wg = 256;  // workgroup size = source tile length
+  int wpc = 4;   // resident workgroups per CU of the persistent grid
   int n_units = 0;
   int grid = 0;
   std::vector<int> begin, end;
@@ -224,8 +233,8 @@ bool batch_live(Runtime& rt, const cvtx_hip_batch* b) {
   return true;
 }
 
-bool build_units(cvtx_hip_batch* b, int T) {
-  const int per = 256 * T;
+bool build_units(cvtx_hip_batch* b, int T, int wg) {
+  const int per = wg * T;
   std::vector<int> order(b->n_clouds);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -236,6 +245,7 @@ bool build_units(cvtx_hip_batch* b, int T) {
     for (int t = b->begin[c]; t < b->end[c]; t += per) units.push_back(make_int2(c, t));
   b->n_units = (int)units.size();
   b->T = T;
+  b->wg = wg;
   if (b->d_units) (void)hipFree(b->d_units);
   b->d_units = nullptr;
   if (units.empty()) return true;
@@ -265,7 +275,7 @@ P3DKParams base_params(cvtx_hip_batch* b, const float* pv, const float* a, float
 hipError_t enqueue_eval(cvtx_hip_batch* b, const float* pv, const float* a, float* vel,
                         float* dvort, float* visc, hipStream_t s) {
   if (b->n_units == 0) return hipSuccess;
-  BFn k = b_pick(b->reg, b->mask, b->acc, b->T);
+  BFn k = b_pick(b->reg, b->mask, b->acc, b->wg, b->T);
   if (!k) return hipErrorInvalidValue;
   BatchParams bp{};
   bp.base = base_params(b, pv, a, vel, dvort, visc);
@@ -278,8 +288,26 @@ hipError_t enqueue_eval(cvtx_hip_batch* b, const float* pv, const float* a, floa
   hipLaunchKernelGGL(batch_reset_counter, dim3(1), dim3(64), 0, s, b->d_counter);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3(b->grid), dim3(256), 0, s, bp);
+  hipLaunchKernelGGL(k, dim3(b->grid), dim3(b->wg), 0, s, bp);
   return hipGetLastError();
+}
+
+// What the autotuner's cache keys a batch by: regularisation, outputs,
+// accumulation, and the buckets (floor log2) of its total pair count and of
+// its number of clouds.
+BatchSignature batch_signature(const cvtx_hip_batch* b) {
+  long long pairs = 0;
+  for (int c = 0; c < b->n_clouds; ++c) {
+    const long long n = b->end[c] - b->begin[c];
+    pairs += n * n;
+  }
+  BatchSignature g;
+  g.reg = b->reg;
+  g.mask = b->mask;
+  g.acc = b->acc;
+  g.pairs = pairs;
+  g.n_clouds = b->n_clouds;
+  return g;
 }
 
 }  // namespace
@@ -344,9 +372,14 @@ cvtx_hip_batch* cvtx_hip_batch_create(int n_clouds, const int* offsets, const fl
     ok = ok && hipMemcpy(b->d_rc, rc.data(), sizeof(RegConsts) * n_clouds, hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemcpy(b->d_sc, sc.data(), sizeof(Scales) * n_clouds, hipMemcpyHostToDevice) == hipSuccess;
   }
-  ok = ok && build_units(b, 2);
+  // the autotuner's shape for this batch's signature, if one is cached;
+  // else two targets per lane, 256-wide workgroups, 4 resident per CU
+  BatchShape shp{2, 256, 4, 0.f};
+  (void)batch_shape_lookup(batch_signature(b), &shp);
+  b->wpc = shp.wpc;
+  ok = ok && build_units(b, shp.T, shp.wg);
   // a few resident workgroups per CU; the loop exits when the units run out
-  b->grid = std::max(1, std::min(b->n_units, 4 * std::max(1, d->n_cu)));
+  b->grid = std::max(1, std::min(b->n_units, b->wpc * std::max(1, d->n_cu)));
   if (!ok) {
     set_error(rt, "batch: device allocation failed");
     batch_free(b);
@@ -370,10 +403,11 @@ int cvtx_hip_batch_set_shape(cvtx_hip_batch* b, int targets_per_lane, int workgr
     set_error(rt, "batch: invalid shape (or RK4 graph already captured)");
     return -1;
   }
-  if (!build_units(b, targets_per_lane)) {
+  if (!build_units(b, targets_per_lane, b->wg)) {
     set_error(rt, "batch: allocation failed");
     return -1;
   }
+  b->wpc = workgroups_per_cu;
   b->grid = std::max(1, std::min(b->n_units, workgroups_per_cu * std::max(1, b->dev->n_cu)));
   return 0;
 }
@@ -524,6 +558,86 @@ int cvtx_hip_batch_rk4_step(cvtx_hip_batch* b, int n_steps, void* stream) {
   hipStream_t s = stream ? (hipStream_t)stream : b->dev->stream;
   for (int i = 0; i < n_steps; ++i)
     if (!check_hip(rt, hipGraphLaunch(b->exec, s), "graph launch")) return -1;
+  return 0;
+}
+
+// Times the persistent launch of this batch (its outputs are overwritten)
+// for every candidate shape -- targets per lane {1, 2, 4} x workgroup size
+// {128, 256} x resident workgroups per CU {2, 4, 8} -- with HIP events on
+// the library's stream, keeps the fastest, and records it in the shape cache
+// under the batch's signature, so later batches of the same signature start
+// with it.  Returns 0 and the winner, or -1.
+int cvtx_hip_batch_autotune(cvtx_hip_batch* b, const float* pv, const float* a, float* vel,
+                            float* dvort, float* visc, int reps, int* targets_per_lane,
+                            int* workgroup_size, int* workgroups_per_cu, float* best_ms) {
+  Runtime& rt = runtime();
+  std::lock_guard<std::recursive_mutex> lk(rt.mu);
+  if (!batch_live(rt, b)) return -1;
+  if (reps < 1 || b->exec || (b->n_particles > 0 && (!pv || !a || !vel || !dvort)) ||
+      ((b->mask & OUT_VISC) && b->n_particles > 0 && !visc)) {
+    set_error(rt, "batch autotune: null buffer, reps < 1, or RK4 graph already captured");
+    return -1;
+  }
+  hipStream_t s = b->dev->stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!check_hip(rt, hipEventCreate(&e0), "event") || !check_hip(rt, hipEventCreate(&e1), "event")) {
+    if (e0) (void)hipEventDestroy(e0);
+    return -1;
+  }
+  BatchShape best{b->T, b->wg, b->wpc, -1.f};
+  bool ok = true;
+  for (int wg : {128, 256}) {
+    for (int T : {1, 2, 4}) {
+      if (!build_units(b, T, wg)) {
+        ok = false;
+        break;
+      }
+      for (int wpc : {2, 4, 8}) {
+        b->grid = std::max(1, std::min(b->n_units, wpc * std::max(1, b->dev->n_cu)));
+        float ms = -1.f;
+        if (enqueue_eval(b, pv, a, vel, dvort, visc, s) == hipSuccess &&  // warm-up
+            hipEventRecord(e0, s) == hipSuccess) {
+          bool run = true;
+          for (int r = 0; r < reps && run; ++r)
+            run = enqueue_eval(b, pv, a, vel, dvort, visc, s) == hipSuccess;
+          if (run && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+              hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+            ms /= (float)reps;
+          else
+            ms = -1.f;
+        }
+        if (ms > 0.f && (best.ms < 0.f || ms < best.ms)) best = BatchShape{T, wg, wpc, ms};
+      }
+    }
+    if (!ok) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // leave the batch in the winning shape (or its previous one)
+  if (!build_units(b, best.T, best.wg)) {
+    set_error(rt, "batch autotune: allocation failed");
+    return -1;
+  }
+  b->wpc = best.wpc;
+  b->grid = std::max(1, std::min(b->n_units, b->wpc * std::max(1, b->dev->n_cu)));
+  if (!ok || best.ms < 0.f) {
+    set_error(rt, "batch autotune: no candidate ran");
+    return -1;
+  }
+  batch_shape_record(batch_signature(b), best);
+  if (targets_per_lane) *targets_per_lane = best.T;
+  if (workgroup_size) *workgroup_size = best.wg;
+  if (workgroups_per_cu) *workgroups_per_cu = best.wpc;
+  if (best_ms) *best_ms = best.ms;
+  return 0;
+}
+
+int cvtx_hip_batch_shape(const cvtx_hip_batch* b, int* targets_per_lane, int* workgroup_size,
+                         int* workgroups_per_cu) {
+  if (!b) return -1;
+  if (targets_per_lane) *targets_per_lane = b->T;
+  if (workgroup_size) *workgroup_size = b->wg;
+  if (workgroups_per_cu) *workgroups_per_cu = b->wpc;
   return 0;
 }
 

@@ -6,7 +6,9 @@
 //   (kind, regularisation, outputs, accumulation, log2 n_src, log2 n_tgt).
 // choose_cfg() consults the cache before falling back to default_cfg().
 // The cache can be saved to and loaded from a small text file, one entry
-// per line: "kind reg mask acc bucket_src bucket_tgt T n_split ms".
+// per line: "kind reg mask acc bucket_src bucket_tgt T n_split ms wg" (kind 3:
+// a multi-cloud batch, keyed by its pair count and cloud count; n_split is
+// then the resident workgroups per CU).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,10 +28,13 @@ namespace cvtx {
 namespace {
 
 using Key = std::tuple<int, int, int, int, int, int>;
+// n_split is the number of source partitions, or for KIND_BATCH the
+// resident workgroups per CU; wg the workgroup size.
 struct Entry {
   int T;
   int n_split;
   float ms;
+  int wg = 256;
 };
 
 std::map<Key, Entry>& cache() {
@@ -111,6 +116,31 @@ LaunchCfg choose_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt,
 
 void record_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt, int T, int n_split) {
   cache()[Key{kind, reg, mask, acc, size_bucket(n_src), size_bucket(n_tgt)}] = Entry{T, n_split, 0.f};
+}
+
+namespace {
+int bucket64(long long n) {
+  int b = 0;
+  while (n > 1) {
+    n >>= 1;
+    ++b;
+  }
+  return b;
+}
+Key batch_key(const BatchSignature& g) {
+  return Key{KIND_BATCH, g.reg, g.mask, g.acc, bucket64(g.pairs), size_bucket(g.n_clouds)};
+}
+}  // namespace
+
+bool batch_shape_lookup(const BatchSignature& g, BatchShape* out) {
+  auto it = cache().find(batch_key(g));
+  if (it == cache().end()) return false;
+  *out = BatchShape{it->second.T, it->second.wg, it->second.n_split, it->second.ms};
+  return true;
+}
+
+void batch_shape_record(const BatchSignature& g, const BatchShape& s) {
+  cache()[batch_key(g)] = Entry{s.T, s.wpc, s.ms, s.wg};
 }
 
 }  // namespace cvtx
@@ -218,9 +248,9 @@ int cvtx_hip_tune_save(const char* path) {
   if (!f) return -1;
   for (auto& kv : cache()) {
     const Key& k = kv.first;
-    std::fprintf(f, "%d %d %d %d %d %d %d %d %.6f\n", std::get<0>(k), std::get<1>(k),
+    std::fprintf(f, "%d %d %d %d %d %d %d %d %.6f %d\n", std::get<0>(k), std::get<1>(k),
                  std::get<2>(k), std::get<3>(k), std::get<4>(k), std::get<5>(k), kv.second.T,
-                 kv.second.n_split, kv.second.ms);
+                 kv.second.n_split, kv.second.ms, kv.second.wg);
   }
   std::fclose(f);
   return 0;
@@ -229,13 +259,17 @@ int cvtx_hip_tune_save(const char* path) {
 int cvtx_hip_tune_load(const char* path) {
   FILE* f = std::fopen(path, "r");
   if (!f) return -1;
-  int kind, reg, mask, acc, bs, bt, T, ns;
+  int kind, reg, mask, acc, bs, bt, T, ns, wg;
   float ms;
   int n = 0;
-  while (std::fscanf(f, "%d %d %d %d %d %d %d %d %f", &kind, &reg, &mask, &acc, &bs, &bt, &T, &ns,
-                     &ms) == 9) {
-    if ((T == 1 || T == 2 || T == 4) && ns >= 1) {
-      cache()[Key{kind, reg, mask, acc, bs, bt}] = Entry{T, ns, ms};
+  char line[256];
+  // one entry per line; the workgroup size column may be absent (256)
+  while (std::fgets(line, sizeof(line), f)) {
+    wg = 256;
+    const int got = std::sscanf(line, "%d %d %d %d %d %d %d %d %f %d", &kind, &reg, &mask, &acc,
+                                &bs, &bt, &T, &ns, &ms, &wg);
+    if (got >= 9 && (T == 1 || T == 2 || T == 4) && ns >= 1 && (wg == 128 || wg == 256)) {
+      cache()[Key{kind, reg, mask, acc, bs, bt}] = Entry{T, ns, ms, wg};
       ++n;
     }
   }

@@ -202,6 +202,23 @@ class Batch:
         self._ck(lib().cvtx_hip_batch_info(self.h, ctypes.byref(u), ctypes.byref(g), ctypes.byref(t)), "info")
         return u.value, g.value, t.value
 
+    def shape(self):
+        """(targets per lane, workgroup size, resident workgroups per CU)."""
+        t, w, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._ck(lib().cvtx_hip_batch_shape(self.h, ctypes.byref(t), ctypes.byref(w),
+                                            ctypes.byref(g)), "shape")
+        return t.value, w.value, g.value
+
+    def autotune(self, pv, a, vel, dvort, visc=None, reps=1):
+        """cvtx_hip_batch_autotune: (targets per lane, workgroup size,
+        workgroups per CU, ms) of the fastest shape, now the batch's."""
+        t, w, g, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        self._ck(lib().cvtx_hip_batch_autotune(self.h, pv.addr, a.addr, vel.addr, dvort.addr,
+                                               visc.addr if visc is not None else None, reps,
+                                               ctypes.byref(t), ctypes.byref(w), ctypes.byref(g),
+                                               ctypes.byref(ms)), "autotune")
+        return t.value, w.value, g.value, ms.value
+
     def eval(self, pv, a, vel, dvort, visc=None, stream=None):
         self._ck(lib().cvtx_hip_batch_eval(self.h, pv.addr, a.addr, vel.addr, dvort.addr,
                                            visc.addr if visc is not None else None, stream), "batch_eval")

@@ -297,9 +297,26 @@ cvtx_hip_batch* cvtx_hip_batch_create(int n_clouds, const int* offsets, const fl
                                       int regularisation, int outputs, int accumulate, float nu);
 void cvtx_hip_batch_destroy(cvtx_hip_batch* batch);
 /* Targets per lane (1, 2, 4) and resident workgroups per CU of the
- * persistent launch; must precede cvtx_hip_batch_rk4_init. */
+ * persistent launch (the workgroup size is kept); must precede
+ * cvtx_hip_batch_rk4_init. */
 int cvtx_hip_batch_set_shape(cvtx_hip_batch* batch, int targets_per_lane, int workgroups_per_cu);
 int cvtx_hip_batch_info(const cvtx_hip_batch* batch, int* n_units, int* grid, int* targets_per_lane);
+/* The batch's launch shape: targets per lane, workgroup size (128 or 256,
+ * also the source tile length) and resident workgroups per CU. */
+int cvtx_hip_batch_shape(const cvtx_hip_batch* batch, int* targets_per_lane, int* workgroup_size,
+                         int* workgroups_per_cu);
+/* Autotuner of the persistent launch: times every candidate shape
+ * (targets per lane {1, 2, 4} x workgroup size {128, 256} x resident
+ * workgroups per CU {2, 4, 8}; one warm-up and `reps` timed evaluations
+ * each, on the library's stream; the outputs are overwritten), keeps the
+ * fastest for this batch and records it in the shape cache under the
+ * batch's signature (regularisation, outputs, accumulation, log2 of the
+ * pair count, log2 of the cloud count): batches created later with the same
+ * signature start in that shape, and cvtx_hip_tune_save/load persist it.
+ * Must precede cvtx_hip_batch_rk4_init.  Returns 0 and the winner, or -1. */
+int cvtx_hip_batch_autotune(cvtx_hip_batch* batch, const float* pv, const float* a, float* vel,
+                            float* dvort, float* visc, int reps, int* targets_per_lane,
+                            int* workgroup_size, int* workgroups_per_cu, float* best_ms);
 /* One persistent launch over all clouds: outputs float[3 * n_particles]. */
 int cvtx_hip_batch_eval(cvtx_hip_batch* batch, const float* pv, const float* a, float* vel,
                         float* dvort, float* visc, void* stream);

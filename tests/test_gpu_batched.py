@@ -184,3 +184,96 @@ def test_rk4_moves_particles():
     z = pv.download()[:, 2]
     assert abs(z.mean() - 10 * 0.01 * uz) < 1e-3 * abs(10 * 0.01 * uz)
     assert z.std() < 1e-4 * abs(z.mean())
+
+
+def test_batch_autotune_records_its_shape(tmp_path):
+    # the library's autotuner picks a shape, leaves the batch in it, and a
+    # new batch of the same signature starts in it; the tuned evaluation
+    # equals the per-cloud launches
+    p, off, sig = small_clouds(16)
+    n = len(p)
+    pv, a = _dev(p)
+    cv.lib().cvtx_hip_tune_clear()
+    try:
+        b = D.Batch(off, sig, "winckelmans")
+        v, d = D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
+        case("batch autotune: 16 clouds")
+        T, wg, wpc, ms = b.autotune(pv, a, v, d, reps=1)
+        assert T in (1, 2, 4) and wg in (128, 256) and wpc in (2, 4, 8) and ms > 0
+        assert b.shape() == (T, wg, wpc)
+        b.eval(pv, a, v, d)
+        v0, d0 = D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
+        b.eval_per_cloud(pv, a, v0, d0)
+        assert_parity("tuned vel", v.download(), v0.download(), 1e-6)
+        assert_parity("tuned dvort", d.download(), d0.download(), 1e-6)
+        b2 = D.Batch(off, sig, "winckelmans")
+        assert b2.shape() == (T, wg, wpc)
+        path = str(tmp_path / "shapes.txt")
+        assert cv.lib().cvtx_hip_tune_save(path.encode()) == 0
+        cv.lib().cvtx_hip_tune_clear()
+        assert cv.lib().cvtx_hip_tune_load(path.encode()) >= 1
+        assert D.Batch(off, sig, "winckelmans").shape() == (T, wg, wpc)
+        b.destroy()
+        b2.destroy()
+    finally:
+        cv.lib().cvtx_hip_tune_clear()
+
+
+@pytest.mark.parametrize("T", (1, 2, 4))
+def test_batch_128_wide_workgroups(tmp_path, T):
+    # a cached shape with 128-wide workgroups (128-source tiles): same results
+    p, off, sig = small_clouds(10)
+    n = len(p)
+    sizes = np.diff(off).astype(np.int64)
+    pairs = int((sizes * sizes).sum())
+    line = f"3 1 3 1 {pairs.bit_length() - 1} {len(sig).bit_length() - 1} {T} 4 0.0 128\n"
+    path = tmp_path / "shape128.txt"
+    path.write_text(line)
+    cv.lib().cvtx_hip_tune_clear()
+    try:
+        assert cv.lib().cvtx_hip_tune_load(str(path).encode()) == 1
+        b = D.Batch(off, sig, "winckelmans")
+        assert b.shape() == (T, 128, 4)
+        pv, a = _dev(p)
+        case(f"batch with 128-wide workgroups, T={T}")
+        v, d = D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
+        b.eval(pv, a, v, d)
+        v0, d0 = D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
+        b.eval_per_cloud(pv, a, v0, d0)
+        assert_parity("128-wide vel", v.download(), v0.download(), 1e-6)
+        assert_parity("128-wide dvort", d.download(), d0.download(), 1e-6)
+        b.destroy()
+    finally:
+        cv.lib().cvtx_hip_tune_clear()
+
+
+def test_batch_rejected_after_finalise():
+    # a batch made before cvtx_finalise refers to freed device state: its
+    # calls fail cleanly (no launch) until it is destroyed
+    p, off, sig = small_clouds(4)
+    n = len(p)
+    pv, a = _dev(p)
+    b = D.Batch(off, sig, "winckelmans")
+    cv.finalise()
+    try:
+        v, d = D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
+        with pytest.raises(cv.CvtxError, match="finalise"):
+            b.eval(pv, a, v, d)
+    finally:
+        cv.initialise()
+        b.destroy()
+    b2 = D.Batch(off, sig, "winckelmans")
+    v, d = D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
+    b2.eval(pv, a, v, d)
+    b2.destroy()
+
+
+def test_per_cloud_visc_batch_without_visc_buffer_is_an_error():
+    p, off, sig = small_clouds(3)
+    n = len(p)
+    pv, a = _dev(p)
+    b = D.Batch(off, sig, "winckelmans", outputs=cv.OUT_VEL | cv.OUT_DVORT | cv.OUT_VISC, nu=1e-3)
+    v, d = D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
+    with pytest.raises(cv.CvtxError, match="null"):
+        b.eval_per_cloud(pv, a, v, d, visc=None)
+    b.destroy()
