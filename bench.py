@@ -17,6 +17,8 @@ Other workloads (--workload):
            from a captured HIP graph
   sheet    N = 2^22 2D vortex sheet, planetary sigma = 4/sqrt(N), P2D vel
            (+ the viscous term, identically zero for planetary)
+  small    per-call latency of the single-source / single-target entry
+           points on the CPU leg and on the device
 
 Each prints one JSON line with: interactions/s (pairs per second of the
 timed work), the roofline of the pair kernel (algorithmic FP32 flops per
@@ -44,6 +46,7 @@ HBM_TBPS = 8.0
 # transcendental 1), fixed per kernel, output set and regularisation.
 FLOPS_PER_PAIR = {
     ("p3d", "vel", "winckelmans"): 31, ("p3d", "vel", "gaussian"): 48,
+    ("p3d", "dvort", "gaussian"): 64,
     ("p3d", "vel+dvort", "singular"): 56, ("p3d", "vel+dvort", "planetary"): 58,
     ("p3d", "vel+dvort", "winckelmans"): 61, ("p3d", "vel+dvort", "gaussian"): 79,
     ("p3d", "vel+dvort+visc", "winckelmans"): 77, ("p3d", "vel+dvort+visc", "gaussian"): 94,
@@ -176,6 +179,19 @@ def bench_ring(args, cv, D, W, o, stream):
                                       "tflops": roofline(pairs, F3, k3)["achieved"],
                                       "frac_of_peak": roofline(pairs, F3, k3)["frac"]}
         visc = o3[2].download()
+    if "single" in args.variants:
+        # the velocity and the stretching alone (a caller of the reference
+        # API makes two calls), device-resident, each with its own kernel
+        for name, outs_, key in (("vel_only", (True, False, False), "vel"),
+                                 ("dvort_only", (False, True, False), "dvort")):
+            o4 = dev.outputs(*outs_)
+            s4, k4 = tm.run(runner(o4, cv.ACC_COMP, False), max(1, args.steps - 1), 1)
+            F4 = FLOPS_PER_PAIR[("p3d", "vel" if key == "vel" else "dvort", "gaussian")]
+            variants[name] = {"interactions_per_s": pairs / (s4 * 1e-3), "kernel_ms": k4,
+                              "flops_per_pair": F4, "frac_of_peak": roofline(pairs, F4, k4)["frac"]}
+            log(f"  {name}: {k4:.1f} ms")
+        variants["vel_then_dvort_over_fused"] = (variants["vel_only"]["kernel_ms"] +
+                                                 variants["dvort_only"]["kernel_ms"]) / ms_k
     if "c_api" in args.variants:
         # the host-array boundary: gather from the caller's pointer arrays,
         # copies over PCIe both ways, kernels, all inside the timed call
@@ -712,9 +728,101 @@ def bench_remesh(args, cv, D, W, o, stream):
     return line
 
 
+# --------------------------------------------------------------------------
+# single-source / single-target forms (S2S, S2M, M2S): latency per call
+# --------------------------------------------------------------------------
+def bench_small(args, cv, D, W, o, stream):
+    """Per-call latency of the reference's single-source and single-target
+    entry points through the public C ABI, on the library's CPU leg and on
+    the device (cvtx_hip_set_dispatch), next to the oracle's restatement of
+    the reference CPU code.  Each figure is the median over batches of
+    calls made from Python through ctypes; `ctypes_call_us` is the cost of a
+    trivial library call the same way, included in every figure."""
+    import ctypes
+    from cvortex_amd.types import P3D, F3D, P2D, V2f, V3f, REG_ID
+    L = cv.lib()
+    rng = np.random.default_rng(0x5EED + 9)
+
+    def per_call_us(fn, calls):
+        best = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            for _ in range(calls):
+                fn()
+            best.append((time.perf_counter() - t0) / calls * 1e6)
+        return float(np.median(best))
+
+    base_us = per_call_us(lambda: L.cvtx_hip_get_dispatch(), 20000)
+    p = W.random_p3d(2, 11, box=1.0, vort=1.0)
+    f = W.random_f3d(1, 12, box=1.0, strength=1.0)
+    q = W.random_p2d(2, 13, box=1.0, vort=1.0)
+    rp = [P3D.from_buffer_copy(np.ascontiguousarray(p[i]).tobytes()) for i in range(2)]
+    rf = F3D.from_buffer_copy(np.ascontiguousarray(f[0]).tobytes())
+    rq = [P2D.from_buffer_copy(np.ascontiguousarray(q[i]).tobytes()) for i in range(2)]
+    mes3 = V3f((ctypes.c_float * 3)(0.3, 0.4, 0.5))
+    mes2 = V2f((ctypes.c_float * 2)(0.3, 0.4))
+    vf = cv.vortfunc("winckelmans")
+    kp = ctypes.byref(vf)
+    w = REG_ID["winckelmans"]
+    s2s = {
+        "P3D_S2S_vel": (lambda: L.cvtx_P3D_S2S_vel(ctypes.byref(rp[0]), mes3, kp, 0.1),
+                        lambda: o.L.orc_P3D_S2S_vel(ctypes.byref(rp[0]), mes3, w, 0.1)),
+        "P3D_S2S_dvort": (lambda: L.cvtx_P3D_S2S_dvort(ctypes.byref(rp[0]), ctypes.byref(rp[1]), kp, 0.1),
+                          lambda: o.L.orc_P3D_S2S_dvort(ctypes.byref(rp[0]), ctypes.byref(rp[1]), w, 0.1)),
+        "P3D_S2S_visc_dvort": (lambda: L.cvtx_P3D_S2S_visc_dvort(ctypes.byref(rp[0]), ctypes.byref(rp[1]), kp, 0.1, 0.01),
+                               lambda: o.L.orc_P3D_S2S_visc_dvort(ctypes.byref(rp[0]), ctypes.byref(rp[1]), w, 0.1, 0.01)),
+        "F3D_S2S_vel": (lambda: L.cvtx_F3D_S2S_vel(ctypes.byref(rf), mes3),
+                        lambda: o.L.orc_F3D_S2S_vel(ctypes.byref(rf), mes3)),
+        "P2D_S2S_vel": (lambda: L.cvtx_P2D_S2S_vel(ctypes.byref(rq[0]), mes2, kp, 0.1),
+                        lambda: o.L.orc_P2D_S2S_vel(ctypes.byref(rq[0]), mes2, w, 0.1)),
+    }
+    out = {"ctypes_call_us": base_us, "s2s": {}, "s2m_m2s": {}}
+    for name, (lib_fn, ref_fn) in s2s.items():
+        row = {}
+        for mode, label in ((cv.DISPATCH_AUTO, "library_auto_us"), (cv.DISPATCH_DEVICE, "library_device_us")):
+            cv.set_dispatch(mode)
+            lib_fn()
+            row[label] = per_call_us(lib_fn, 2000 if mode == cv.DISPATCH_AUTO else 100)
+            if mode == cv.DISPATCH_AUTO:
+                row["leg"] = "host" if cv.last_leg() == cv.LEG_HOST else "device"
+        cv.set_dispatch(cv.DISPATCH_AUTO)
+        row["oracle_cpu_us"] = per_call_us(ref_fn, 2000)
+        out["s2s"][name] = row
+        log(f"  {name}: auto {row['library_auto_us']:.2f} us ({row['leg']}), device "
+            f"{row['library_device_us']:.1f} us, oracle {row['oracle_cpu_us']:.2f} us")
+    # S2M (one source, n targets) and M2S (n sources, one target): both legs
+    for n in (1024, 16384, 65536, 262144, 1 << 20):
+        src = W.random_p3d(n, 20 + n.bit_length(), box=1.0, vort=1.0)
+        pts = np.ascontiguousarray(src["coord"])
+        rows = {}
+        for mode, label in ((cv.DISPATCH_HOST, "host_ms"), (cv.DISPATCH_DEVICE, "device_ms")):
+            cv.set_dispatch(mode)
+            cv.P3D_S2M_vel(src[0], pts, "winckelmans", 0.05)
+            cv.P3D_M2S_vel(src, pts[0], "winckelmans", 0.05)
+            reps = 3
+            rows.setdefault("S2M_vel", {})[label] = per_call_us(
+                lambda: cv.P3D_S2M_vel(src[0], pts, "winckelmans", 0.05), reps) / 1e3
+            rows.setdefault("M2S_vel", {})[label] = per_call_us(
+                lambda: cv.P3D_M2S_vel(src, pts[0], "winckelmans", 0.05), reps) / 1e3
+        cv.set_dispatch(cv.DISPATCH_AUTO)
+        cv.P3D_M2S_vel(src, pts[0], "winckelmans", 0.05)
+        rows["auto_leg"] = "host" if cv.last_leg() == cv.LEG_HOST else "device"
+        out["s2m_m2s"][str(n)] = rows
+        log(f"  n={n}: S2M host {rows['S2M_vel']['host_ms']:.3f} ms, device "
+            f"{rows['S2M_vel']['device_ms']:.3f} ms; M2S host {rows['M2S_vel']['host_ms']:.3f} ms, "
+            f"device {rows['M2S_vel']['device_ms']:.3f} ms; auto: {rows['auto_leg']}")
+    s2s_auto = out["s2s"]["P3D_S2S_vel"]["library_auto_us"]
+    line = base_line(args, 1e6 / s2s_auto, "calls/s", s2s_auto * 1e-3,
+                     {"workload": "single-source / single-target entry points (S2S, S2M, M2S) "
+                                  "through the public C ABI, Winckelmans sigma 0.1 / 0.05",
+                      "host_threads": cv.host_threads()})
+    line.update({"roofline": None, "latency": out, "cpu_baseline": None})
+    return line
+
+
 WORKLOADS = {"ring": bench_ring, "cube": bench_cube, "lattice": bench_lattice,
              "clouds": bench_clouds, "sheet": bench_sheet, "relax": bench_relax,
-             "influence": bench_influence, "remesh": bench_remesh}
+             "influence": bench_influence, "remesh": bench_remesh, "small": bench_small}
 
 
 def main():
@@ -729,7 +837,7 @@ def main():
     ap.add_argument("--cpu-samples", type=int, default=512)
     ap.add_argument("--rk4-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variants", default="plain,visc,c_api,fp64final,gaussian")
+    ap.add_argument("--variants", default="plain,visc,single,c_api,fp64final,gaussian")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

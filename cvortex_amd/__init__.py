@@ -30,7 +30,7 @@ OUT_VEL, OUT_DVORT, OUT_VISC = 1, 2, 4
 DISPATCH_AUTO, DISPATCH_DEVICE, DISPATCH_HOST = 0, 1, 2
 LEG_DEVICE, LEG_HOST = 0, 1
 # single-source / single-target calls from this many elements go to the device
-SMALL_DEVICE_MIN = 65536
+SMALL_DEVICE_MIN = 4096
 
 _lib = None
 

@@ -20,6 +20,7 @@
 
 #include <cvortex_hip.h>
 
+#include "api_common.hpp"
 #include "runtime.hpp"
 #include "tune.hpp"
 
@@ -68,8 +69,20 @@ int size_bucket(int n) {
   return b;
 }
 
+// Whether a launch of this kind orders its targets spatially (api.cpp), which
+// changes the cost of its launch shapes: part of the cache key, folded into
+// the kind field (kind + 16).
+int ordered_kind(int kind, int reg, int n_src, int n_tgt) {
+  const Runtime& rt = runtime();
+  const bool ordered =
+      rt.order_targets && ((kind == KIND_P3D && p3d_wants_order(reg, n_src, n_tgt)) ||
+                           (kind == KIND_F3D && f3d_wants_order(n_src, n_tgt)));
+  return ordered ? kind + 16 : kind;
+}
+
 LaunchCfg choose_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt, int n_cu) {
-  auto it = cache().find(Key{kind, reg, mask, acc, size_bucket(n_src), size_bucket(n_tgt)});
+  auto it = cache().find(Key{ordered_kind(kind, reg, n_src, n_tgt), reg, mask, acc,
+                             size_bucket(n_src), size_bucket(n_tgt)});
   if (it != cache().end()) return launch_shape(it->second.T, it->second.n_split, n_src);
   const int cus = n_cu > 0 ? n_cu : 256;
   // Spatially ordered targets (Gaussian P3D, compensated sums).  One target
@@ -90,12 +103,17 @@ LaunchCfg choose_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt,
   // The plain FP32 kernel has no FP64 branch and keeps the general shape
   // (two targets per lane: 678 ms against 861 ms with one at 2^20).
   if (kind == KIND_P3D && acc == ACC_COMP && p3d_wants_order(reg, n_src, n_tgt)) {
-    const long long blocks = ((long long)n_tgt + 255) / 256;
+    // The velocity alone has no FP64 near sums, so the near branch costs
+    // little and four targets per lane share each source's LDS reads: four
+    // targets per lane and partitions for about 256 workgroups per CU (ring
+    // 2^20 vel: 475.7 ms with the rule below, 419.6 ms with this).
+    const int T = mask == OUT_VEL ? 4 : 1;
+    const long long blocks = ((long long)n_tgt + 256LL * T - 1) / (256LL * T);
     const long long few = (64LL * cus + blocks - 1) / blocks;
     const long long per = ((long long)n_src + 65535) / 65536;
     const long long many = (256LL * cus + blocks - 1) / blocks;
-    const long long k = std::min(std::max(few, per), many);
-    return launch_shape(1, (int)std::min<long long>(k, 1 << 20), n_src);
+    const long long k = T == 4 ? many : std::min(std::max(few, per), many);
+    return launch_shape(T, (int)std::min<long long>(k, 1 << 20), n_src);
   }
   LaunchCfg c = default_cfg(n_src, n_tgt, n_cu);
   const bool many_blocks = (long long)(n_tgt + 255) / 256 >= 4LL * cus;
@@ -115,7 +133,8 @@ LaunchCfg choose_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt,
 }
 
 void record_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt, int T, int n_split) {
-  cache()[Key{kind, reg, mask, acc, size_bucket(n_src), size_bucket(n_tgt)}] = Entry{T, n_split, 0.f};
+  cache()[Key{ordered_kind(kind, reg, n_src, n_tgt), reg, mask, acc, size_bucket(n_src),
+              size_bucket(n_tgt)}] = Entry{T, n_split, 0.f};
 }
 
 namespace {
@@ -219,6 +238,11 @@ int cvtx_hip_autotune_p3d(const cvtx_hip_p3d_problem* p, int reps, int* best_T, 
   j.nu = p->nu;
   j.gtab = d->gtab;
   if (!p3d_supported(j.reg, j.mask) || j.n_tgt <= 0 || j.n_src < 0) return -1;
+  // time the shapes on the target order the real calls will use
+  if (rt.order_targets && p3d_wants_order(j.reg, j.n_src, j.n_tgt)) {
+    j.tgt_perm = order_points(rt, *d, j.tgt_pv, j.n_tgt, SLOT_PERM, d->stream);
+    if (!j.tgt_perm) return -1;
+  }
   float best = -1.f;
   LaunchCfg win;
   for (const LaunchCfg& c : candidates(j.n_src, j.n_tgt, d->n_cu)) {
@@ -235,8 +259,8 @@ int cvtx_hip_autotune_p3d(const cvtx_hip_p3d_problem* p, int reps, int* best_T, 
     }
   }
   if (best < 0.f) return -1;
-  cache()[Key{KIND_P3D, j.reg, j.mask, j.acc, size_bucket(j.n_src), size_bucket(j.n_tgt)}] =
-      Entry{win.T, win.n_split, best};
+  cache()[Key{ordered_kind(KIND_P3D, j.reg, j.n_src, j.n_tgt), j.reg, j.mask, j.acc,
+              size_bucket(j.n_src), size_bucket(j.n_tgt)}] = Entry{win.T, win.n_split, best};
   if (best_T) *best_T = win.T;
   if (best_split) *best_split = win.n_split;
   if (best_ms) *best_ms = best;

@@ -149,7 +149,7 @@ const char* cvtx_hip_last_error(void);
 #define CVTX_HIP_DISPATCH_AUTO 0
 #define CVTX_HIP_DISPATCH_DEVICE 1
 #define CVTX_HIP_DISPATCH_HOST 2
-#define CVTX_HIP_SMALL_DEVICE_MIN 65536
+#define CVTX_HIP_SMALL_DEVICE_MIN 4096
 int cvtx_hip_set_dispatch(int mode);
 int cvtx_hip_get_dispatch(void);
 /* Leg that evaluated the last public void call: CVTX_HIP_LEG_DEVICE,

@@ -194,7 +194,8 @@ def test_autotuner_records_a_shape_that_agrees(tmp_path):
         path = str(tmp_path / "shapes.txt")
         assert L.cvtx_hip_tune_save(path.encode()) == 0
         fields = open(path).read().split()
-        assert len(fields) == 9 and int(fields[6]) == T and int(fields[7]) == k
+        assert len(fields) == 10 and int(fields[6]) == T and int(fields[7]) == k
+        assert int(fields[9]) == 256
         L.cvtx_hip_tune_clear()
         assert L.cvtx_hip_tune_load(path.encode()) == 1
     finally:
