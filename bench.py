@@ -170,6 +170,17 @@ def bench_ring(args, cv, D, W, o, stream):
         variants["plain_fp32"] = {"interactions_per_s": pairs / (s2 * 1e-3), "kernel_ms": k2,
                                   "frac_of_peak": roofline(pairs, F, k2)["frac"]}
         plain = (o2[0].download(), o2[1].download())
+    tiles = None
+    if "tiles" in args.variants:
+        # FP32 tiles folded by TwoSum for every sum, no per-pair FP64: the
+        # ring's near stretching terms are coherent, so this shows what the
+        # per-pair FP64 sums of the default buy
+        o5 = dev.outputs(True, True, False)
+        s5, k5 = tm.run(runner(o5, cv.ACC_TILES, False), max(1, args.steps - 1), 1)
+        variants["acc_tiles"] = {"interactions_per_s": pairs / (s5 * 1e-3), "kernel_ms": k5,
+                                 "frac_of_peak": roofline(pairs, F, k5)["frac"]}
+        tiles = (o5[0].download(), o5[1].download())
+        log(f"  ACC_TILES: {k5:.1f} ms")
     visc = None
     if "visc" in args.variants:
         o3 = dev.outputs(True, True, True)
@@ -225,6 +236,9 @@ def bench_ring(args, cv, D, W, o, stream):
         if plain is not None:
             err["plain_fp32"] = {"vel": err_stats(plain[0][idx], v64),
                                  "dvort": err_stats(plain[1][idx], d64)}
+        if tiles is not None:
+            err["acc_tiles"] = {"vel": err_stats(tiles[0][idx], v64),
+                                "dvort": err_stats(tiles[1][idx], d64)}
     cpu = None
     if not args.no_cpu:
         m = min(args.cpu_samples, n)
@@ -357,6 +371,9 @@ def bench_lattice(args, cv, D, W, o, stream):
 # --------------------------------------------------------------------------
 # C4: 512 clouds, batched persistent launch, RK4 graph
 # --------------------------------------------------------------------------
+F_CLOUDS = FLOPS_PER_PAIR[("p3d", "vel+dvort", "winckelmans")]
+
+
 def bench_clouds(args, cv, D, W, o, stream):
     n_clouds = args.n or 512
     p, off, sig = W.clouds(n_clouds)
@@ -404,10 +421,11 @@ def bench_clouds(args, cv, D, W, o, stream):
     order = np.argsort(sizes)
     pick = order[np.linspace(0, n_clouds - 1, 8).astype(int)]
     rng = np.random.default_rng(0x5EED + 4)
-    errs_v, errs_d = [], []
+    errs_v, errs_d, idxs = [], [], []
     for c in pick:
         cl = np.ascontiguousarray(p[off[c]:off[c + 1]])
         idx = np.sort(rng.choice(len(cl), size=min(256, len(cl)), replace=False)).astype(np.int32)
+        idxs.append(idx)
         v64, d64, _ = o.f64_P3D(cl, cl, "winckelmans", float(sig[c]), idx=idx)
         errs_v.append((V[off[c]:off[c + 1]][idx], v64))
         errs_d.append((Dv[off[c]:off[c + 1]][idx], d64))
@@ -418,6 +436,26 @@ def bench_clouds(args, cv, D, W, o, stream):
     cpu_pairs = float(sum(len(cl) ** 2 for cl, _ in subset))
     gv = np.concatenate([e[0] for e in errs_v]); rv = np.concatenate([e[1] for e in errs_v])
     gd = np.concatenate([e[0] for e in errs_d]); rd = np.concatenate([e[1] for e in errs_d])
+    tiles = None
+    if "tiles" in args.variants:
+        # FP32 tile sums folded by TwoSum for every sum, no per-pair FP64:
+        # the clouds' strengths are independent, so their near stretching
+        # terms do not add up coherently
+        bt = D.Batch(off, sig, "winckelmans", outputs=cv.OUT_VEL | cv.OUT_DVORT,
+                     accumulate=cv.ACC_TILES)
+        tt = bt.autotune(dpv, da, v2, d2, reps=1)
+        t_step, t_k = tm.run(lambda: bt.eval(dpv, da, v2, d2, stream=stream),
+                             max(1, args.steps - 1), 1)
+        Vt, Dt = v2.download(), d2.download()
+        tv =np.concatenate([Vt[off[c]:off[c + 1]][i] for c, i in zip(pick, idxs)])
+        td = np.concatenate([Dt[off[c]:off[c + 1]][i] for c, i in zip(pick, idxs)])
+        tiles = {"what": "accumulate=CVTX_HIP_ACC_TILES: FP32 tile sums + TwoSum across tiles "
+                         "for every sum, no per-pair FP64 stretching",
+                 "ms_per_step": t_step, "interactions_per_s": pairs / (t_step * 1e-3),
+                 "shape": list(tt[:3]), "roofline": roofline(pairs, F_CLOUDS, t_k),
+                 "rel_err_vs_fp64": {"vel": err_stats(tv, rv), "dvort": err_stats(td, rd)}}
+        log(f"  ACC_TILES {t_step:.1f} ms (shape {tt[:3]})")
+        bt.destroy()
     cpu = None
     if not args.no_cpu:
         best_cpu, _ = cpu_time(lambda: [(o.P3D_M2M_vel(cl, cl["coord"], "winckelmans", s_),
@@ -429,7 +467,6 @@ def bench_clouds(args, cv, D, W, o, stream):
                          f"sizes {min(len(c) for c, _ in subset)}.."
                          f"{max(len(c) for c, _ in subset)}, {cpu_pairs:.3g} "
                          f"pairs, {best_cpu:.2f} s"}
-    F = FLOPS_PER_PAIR[("p3d", "vel+dvort", "winckelmans")]
     line = base_line(args, pairs / (ms_step * 1e-3), "interactions/s", ms_step,
                      {"workload": "512 clouds, sizes log-uniform in [1024, 65536], unit cubes, "
                                   "Winckelmans sigma_c ~ U[0.01, 0.1], fused vel+dvort",
@@ -437,7 +474,7 @@ def bench_clouds(args, cv, D, W, o, stream):
                       "shape": {"targets_per_lane": tuned[0], "workgroup_size": tuned[1],
                                 "workgroups_per_cu": tuned[2], "tuned_by": "cvtx_hip_batch_autotune",
                                 "tuning_s": tune_s, "shape_before_tuning": list(shape0)}})
-    line.update({"roofline": roofline(pairs, F, ms_k),
+    line.update({"roofline": roofline(pairs, F_CLOUDS, ms_k),
                  "per_cloud_plain_launches": {
                      "what": "one launch per cloud, 1 target per lane, no source partitions, "
                              "stream synchronised after each cloud",
@@ -453,6 +490,7 @@ def bench_clouds(args, cv, D, W, o, stream):
                  "rk4_graph": rk,
                  "rel_err_vs_fp64": {"samples": int(len(gv)), "clouds": [int(c) for c in pick],
                                      "vel": err_stats(gv, rv), "dvort": err_stats(gd, rd)},
+                 "acc_tiles": tiles,
                  "cpu_baseline": cpu})
     b.destroy()
     return line
@@ -837,7 +875,7 @@ def main():
     ap.add_argument("--cpu-samples", type=int, default=512)
     ap.add_argument("--rk4-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variants", default="plain,visc,single,c_api,fp64final,gaussian")
+    ap.add_argument("--variants", default="plain,tiles,visc,single,c_api,fp64final,gaussian")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

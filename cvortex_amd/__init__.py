@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcvortex.
 
 ACC_PLAIN = 0
 ACC_COMP = 1
+ACC_TILES = 2
 OUT_VEL, OUT_DVORT, OUT_VISC = 1, 2, 4
 # which leg the public void functions use (cvtx_hip_set_dispatch)
 DISPATCH_AUTO, DISPATCH_DEVICE, DISPATCH_HOST = 0, 1, 2

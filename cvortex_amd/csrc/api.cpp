@@ -533,7 +533,7 @@ const char* cvtx_hip_last_error(void) { return runtime().last_error.c_str(); }
 int cvtx_hip_set_accumulation(int mode) {
   Runtime& rt = runtime();
   Lock lk(rt.mu);
-  if (mode != ACC_PLAIN && mode != ACC_COMP) return -1;
+  if (!valid_acc(mode)) return -1;
   const int prev = rt.acc_mode;
   rt.acc_mode = mode;
   return prev;
@@ -596,7 +596,7 @@ bool p3d_job_from_problem(Runtime& rt, const Device& d, const cvtx_hip_p3d_probl
     set_error(rt, "invalid device problem");
     return false;
   }
-  if (j.acc != ACC_PLAIN && j.acc != ACC_COMP) {
+  if (!valid_acc(j.acc)) {
     set_error(rt, "invalid accumulation mode");
     return false;
   }
@@ -656,7 +656,7 @@ static int p2d_dev(const cvtx_hip_p2d_problem* p, const LaunchCfg* forced, void*
     set_error(rt, "invalid device problem");
     return -1;
   }
-  if ((j.acc != ACC_PLAIN && j.acc != ACC_COMP) || !p2d_supported(j.reg, j.mask)) {
+  if (!valid_acc(j.acc) || !p2d_supported(j.reg, j.mask)) {
     set_error(rt, "unsupported regularisation/output/accumulation combination");
     return -1;
   }
@@ -691,7 +691,7 @@ static int f3d_dev(const cvtx_hip_f3d_problem* p, const LaunchCfg* forced, void*
   j.mask = (p->vel ? OUT_VEL : 0) | (p->dvort ? OUT_DVORT : 0);
   j.acc = p->accumulate;
   if (j.n_fil < 0 || j.n_tgt < 0 || (j.n_fil > 0 && (!j.fa || !j.fb)) || (j.n_tgt > 0 && !j.tgt_pv) ||
-      j.mask == 0 || (j.acc != ACC_PLAIN && j.acc != ACC_COMP)) {
+      j.mask == 0 || !valid_acc(j.acc)) {
     set_error(rt, "invalid device problem");
     return -1;
   }

@@ -133,7 +133,9 @@ BFn b_wg(int wg, int T) {
 }
 template <int REG, int MASK>
 BFn b_acc(int acc, int wg, int T) {
-  return acc == ACC_PLAIN ? b_wg<REG, MASK, ACC_PLAIN>(wg, T) : b_wg<REG, MASK, ACC_COMP>(wg, T);
+  if (acc == ACC_PLAIN) return b_wg<REG, MASK, ACC_PLAIN>(wg, T);
+  if (acc == ACC_TILES) return b_wg<REG, MASK, ACC_TILES>(wg, T);
+  return b_wg<REG, MASK, ACC_COMP>(wg, T);
 }
 template <int REG>
 BFn b_mask(int mask, int wg, int T, int acc) {
@@ -330,7 +332,7 @@ cvtx_hip_batch* cvtx_hip_batch_create(int n_clouds, const int* offsets, const fl
     set_error(rt, "batch: outputs must be vel|dvort or vel|dvort|visc");
     return nullptr;
   }
-  if (accumulate != ACC_PLAIN && accumulate != ACC_COMP) {
+  if (!valid_acc(accumulate)) {
     set_error(rt, "batch: invalid accumulation mode");
     return nullptr;
   }

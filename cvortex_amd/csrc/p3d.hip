@@ -11,7 +11,8 @@
 //   * per pair: d, r2, one regularisation evaluation shared by all outputs,
 //     then FMA accumulation into the reduced sums of device_common.hpp;
 //   * ACC_COMP: per-tile FP32 sums folded into a TwoSum (hi, lo) pair per
-//     accumulator; ACC_PLAIN: one FP32 running sum;
+//     accumulator, FP64 stretching and PSE sums per pair; ACC_TILES: the
+//     folded tile sums for every accumulator; ACC_PLAIN: one FP32 running sum;
 //   * n_split > 1: each grid row handles one source partition and writes FP64
 //     partial sums; p3d_reduce adds the partitions in FP64 and applies the
 //     epilogue.  Otherwise the epilogue runs in the main kernel.
@@ -141,7 +142,9 @@ KernelFn pick_T(int T) {
 
 template <int REG, int MASK>
 KernelFn pick_acc(int acc, int T) {
-  return acc == ACC_PLAIN ? pick_T<REG, MASK, ACC_PLAIN>(T) : pick_T<REG, MASK, ACC_COMP>(T);
+  if (acc == ACC_PLAIN) return pick_T<REG, MASK, ACC_PLAIN>(T);
+  if (acc == ACC_TILES) return pick_T<REG, MASK, ACC_TILES>(T);
+  return pick_T<REG, MASK, ACC_COMP>(T);
 }
 
 template <int REG>

@@ -30,7 +30,9 @@ struct P3DLayout {
 // pair in FP64: their near-field terms are large and nearly parallel to a_t,
 // and only their small component normal to a_t (or the small difference
 // V_t E - a_t e) survives, so FP32 tile sums would lose several digits
-// there.  With ACC_PLAIN everything is FP32.
+// there.  With ACC_PLAIN and ACC_TILES everything is FP32 (ACC_TILES folds
+// the tile sums by TwoSum, which suits fields whose near terms are not
+// coherent, e.g. i.i.d. strengths).
 // SPLIT (Gaussian with ACC_COMP): only pairs with rho^2 < 208 go to FP64,
 // inside the regularisation's near branch; the stretching terms of the
 // pairs beyond (Kv < 208^-3/2 in reduced units, and eta = 0 there) are
@@ -258,7 +260,7 @@ __device__ __forceinline__ void p3d_block(const P3DKParams& p, const RegConsts& 
       }
     }
     __syncthreads();
-    if constexpr (ACC == ACC_COMP) {
+    if constexpr (ACC != ACC_PLAIN) {
 #pragma unroll
       for (int k = 0; k < T; ++k)
 #pragma unroll
@@ -283,7 +285,7 @@ __device__ __forceinline__ void p3d_block(const P3DKParams& p, const RegConsts& 
         p3d_pair<REG, MASK, ACC>(sp, sa, &sm.xy[DBL_ ? jv : 0], &sm.zv[DBL_ ? jv : 0], xt[k], at[k],
                                  rv, sm.tab, acc[k], dacc[k]);
     }
-    if constexpr (ACC == ACC_COMP) {
+    if constexpr (ACC != ACC_PLAIN) {
 #pragma unroll
       for (int k = 0; k < T; ++k)
 #pragma unroll
@@ -298,7 +300,7 @@ __device__ __forceinline__ void p3d_block(const P3DKParams& p, const RegConsts& 
     // gather the canonical layout
     double d[L::N];
     auto fval = [&](int a) -> double {
-      return (ACC == ACC_COMP) ? (double)hi[k][a] + (double)lo[k][a] : (double)acc[k][a];
+      return (ACC != ACC_PLAIN) ? (double)hi[k][a] + (double)lo[k][a] : (double)acc[k][a];
     };
     if constexpr ((MASK & OUT_VEL) != 0) {
 #pragma unroll

@@ -166,7 +166,7 @@ static int vort_dev(const cvtx_hip_vort_problem* p, const LaunchCfg* forced, voi
   if (!p || p->n_src < 0 || p->n_tgt < 0 || (p->n_src > 0 && (!p->src_pv || !p->src_a)) ||
       (p->n_tgt > 0 && (!p->tgt || !p->vort)) || p->regularisation < REG_SINGULAR ||
       p->regularisation > REG_GAUSSIAN ||
-      (p->accumulate != ACC_PLAIN && p->accumulate != ACC_COMP)) {
+      !valid_acc(p->accumulate)) {
     set_error(rt, "invalid vorticity problem");
     return -1;
   }
@@ -199,7 +199,7 @@ int cvtx_hip_relax_eval(const float* pv, float* a, int n, float fdt, int regular
   Runtime& rt = runtime();
   Lock lk(rt.mu);
   if (n < 0 || (n > 0 && (!pv || !a)) || regularisation < REG_SINGULAR ||
-      regularisation > REG_GAUSSIAN || (accumulate != ACC_PLAIN && accumulate != ACC_COMP)) {
+      regularisation > REG_GAUSSIAN || !valid_acc(accumulate)) {
     set_error(rt, "invalid relaxation problem");
     return -1;
   }

@@ -56,7 +56,15 @@ extern "C" {
 #define CVTX_HIP_OUT_VISC 4
 
 #define CVTX_HIP_ACC_PLAIN 0 /* one FP32 running sum per output component */
-#define CVTX_HIP_ACC_COMP 1  /* FP32 tile sums + TwoSum across tiles, FP64 across partitions */
+/* FP32 tile sums + TwoSum across tiles, FP64 across partitions; P3D also sums
+ * the stretching (and PSE) terms of near pairs in FP64 per pair: coherent
+ * fields (a vortex ring) need it, their near terms being large and nearly
+ * parallel to the target's strength */
+#define CVTX_HIP_ACC_COMP 1
+/* FP32 tile sums + TwoSum across tiles for every sum, no per-pair FP64:
+ * enough for fields without that coherence (independent random strengths);
+ * the same as CVTX_HIP_ACC_COMP for P2D, F3D and the vorticity field */
+#define CVTX_HIP_ACC_TILES 2
 
 /* ---- 1. seam: replaces opencl_brute_force_* (reference src/ocl_P3D.h:33-68,
  *         src/ocl_F3D.h, src/ocl_P2D.h) ---- */

@@ -73,6 +73,33 @@ def test_batched_matches_per_cloud_and_oracle(orc, kernel):
     b.destroy()
 
 
+@pytest.mark.parametrize("kernel", ["winckelmans", "gaussian"])
+def test_batched_tiles_accumulation(orc, kernel):
+    # FP32 tile sums folded by TwoSum for every sum, no per-pair FP64
+    p, off, sig = small_clouds(16)
+    n = len(p)
+    case(f"batched tiles accumulation {kernel} n={n}")
+    pv, a = _dev(p)
+    b = D.Batch(off, sig, kernel, outputs=cv.OUT_VEL | cv.OUT_DVORT | cv.OUT_VISC, nu=1e-3,
+                accumulate=cv.ACC_TILES)
+    vel, dv, vs = D.DeviceArray((n, 3)), D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
+    b.eval(pv, a, vel, dv, vs)
+    V, Dv, S = vel.download(), dv.download(), vs.download()
+    vel2, dv2, vs2 = D.DeviceArray((n, 3)), D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
+    b.eval_per_cloud(pv, a, vel2, dv2, vs2)
+    assert_parity("persistent vs per-cloud vel", V, vel2.download(), 1e-6)
+    assert_parity("persistent vs per-cloud dvort", Dv, dv2.download(), 1e-6)
+    assert_parity("persistent vs per-cloud visc", S, vs2.download(), 1e-6)
+    for c in range(0, len(sig), 3):
+        lo_, hi_ = off[c], off[c + 1]
+        cl = np.ascontiguousarray(p[lo_:hi_])
+        v64, d64, s64 = orc.f64_P3D(cl, cl, kernel, float(sig[c]), 1e-3, gauss_mode=1, visc=True)
+        assert_parity(f"cloud {c} vel", V[lo_:hi_], v64, 1e-5)
+        assert_parity(f"cloud {c} dvort", Dv[lo_:hi_], d64, 1e-5)
+        assert_parity(f"cloud {c} visc", S[lo_:hi_], s64, 1e-5)
+    b.destroy()
+
+
 @pytest.mark.parametrize("T,wpc", [(1, 2), (2, 4), (4, 1), (2, 8)])
 def test_batched_shapes(T, wpc):
     p, off, sig = small_clouds(12)

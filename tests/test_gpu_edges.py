@@ -1,7 +1,7 @@
 """Edge cases of the GPU path: empty and tiny inputs, sizes that are not
 multiples of the 256-source tile, aliasing of sources and targets,
 coincident particles, negative regularisation radius, every launch shape,
-plain vs compensated accumulation, and the device-resident API against the
+plain, tile-compensated and compensated accumulation, and the device-resident API against the
 host-array API.  Tolerance: relative L2 <= 1e-5 against the oracle's FP32
 restatement of the reference (written per assertion)."""
 import numpy as np
@@ -119,6 +119,38 @@ def test_plain_accumulation(orc):
             check(orc, k, p, p, 0.1, 0.05, " (plain)")
     finally:
         cv.set_accumulation(prev)
+
+
+def test_tiles_accumulation(orc):
+    # FP32 tile sums folded by TwoSum for every sum, no per-pair FP64
+    case("tiles accumulation")
+    p = W.random_p3d(5000, 62, box=1.0, vort=1.0)
+    prev = cv.set_accumulation(cv.ACC_TILES)
+    try:
+        for k in ("winckelmans", "gaussian", "planetary", "singular"):
+            check(orc, k, p, p, 0.1, 0.05 if k in ("winckelmans", "gaussian") else None,
+                  " (tiles)")
+    finally:
+        cv.set_accumulation(prev)
+
+
+def test_tiles_accumulation_on_a_large_random_cloud(orc):
+    # 2^17 particles with independent strengths: the TwoSum-folded FP32 tile
+    # sums stay within 1e-5 of the FP64 sums, where a single FP32 running sum
+    # drifts further
+    case("tiles accumulation, 2^17 random cloud")
+    n = 1 << 17
+    p = W.random_p3d(n, 63, box=1.0, vort=1.0)
+    idx = np.sort(np.random.default_rng(64).choice(n, size=512, replace=False)).astype(np.int32)
+    v64, d64, _ = orc.f64_P3D(p, p, "winckelmans", 0.05, idx=idx)
+    prev = cv.set_accumulation(cv.ACC_TILES)
+    try:
+        v, d, _ = cv.P3D_M2M_fused(p, p, "winckelmans", 0.05, 0.0, vel=True, dvort=True)
+    finally:
+        cv.set_accumulation(prev)
+    ev, ed = oracle.rel_l2(v[idx], v64), oracle.rel_l2(d[idx], d64)
+    print(f"tiles vs FP64: vel {ev:.3g} dvort {ed:.3g}", flush=True)
+    assert ev <= 1e-5 and ed <= 1e-5, (ev, ed)
 
 
 @pytest.mark.parametrize("n", [20000, 70000])
