@@ -13,8 +13,8 @@ Other workloads (--workload):
   lattice  16384 straight filaments (128 x 128 sheet) on 2^20 particles,
            F3D vel, dvort and fused
   clouds   512 independent clouds (1024..65536 particles), fused vel + dvort,
-           one persistent launch vs one launch per cloud, RK4 steps replayed
-           from a captured HIP graph
+           tile-compensated sums, one persistent launch vs one launch per
+           cloud, RK4 steps replayed from a captured HIP graph
   sheet    N = 2^22 2D vortex sheet, planetary sigma = 4/sqrt(N), P2D vel
            (+ the viscous term, identically zero for planetary)
   small    per-call latency of the single-source / single-target entry
@@ -171,7 +171,7 @@ def bench_ring(args, cv, D, W, o, stream):
                                   "frac_of_peak": roofline(pairs, F, k2)["frac"]}
         plain = (o2[0].download(), o2[1].download())
     tiles = None
-    if "tiles" in args.variants:
+    if "acc" in args.variants:
         # FP32 tiles folded by TwoSum for every sum, no per-pair FP64: the
         # ring's near stretching terms are coherent, so this shows what the
         # per-pair FP64 sums of the default buy
@@ -384,7 +384,12 @@ def bench_clouds(args, cv, D, W, o, stream):
     dpv, da = D.DeviceArray.from_host(pv), D.DeviceArray.from_host(a)
     vel, dv = D.DeviceArray((n, 3)), D.DeviceArray((n, 3))
     tm = Timer(stream)
-    b = D.Batch(off, sig, "winckelmans", outputs=cv.OUT_VEL | cv.OUT_DVORT, accumulate=cv.ACC_COMP)
+    # the clouds' strengths are independent random vectors, so their near
+    # stretching terms do not add up coherently: FP32 tile sums folded by
+    # TwoSum (CVTX_HIP_ACC_TILES) hold the same error bar as the per-pair
+    # FP64 sums of CVTX_HIP_ACC_COMP (both are reported)
+    b = D.Batch(off, sig, "winckelmans", outputs=cv.OUT_VEL | cv.OUT_DVORT,
+                accumulate=cv.ACC_TILES)
     # the library's autotuner picks the persistent launch shape (targets per
     # lane x workgroup size x resident workgroups per CU) and caches it
     shape0 = b.shape()
@@ -436,25 +441,23 @@ def bench_clouds(args, cv, D, W, o, stream):
     cpu_pairs = float(sum(len(cl) ** 2 for cl, _ in subset))
     gv = np.concatenate([e[0] for e in errs_v]); rv = np.concatenate([e[1] for e in errs_v])
     gd = np.concatenate([e[0] for e in errs_d]); rd = np.concatenate([e[1] for e in errs_d])
-    tiles = None
-    if "tiles" in args.variants:
-        # FP32 tile sums folded by TwoSum for every sum, no per-pair FP64:
-        # the clouds' strengths are independent, so their near stretching
-        # terms do not add up coherently
+    comp = None
+    if "acc" in args.variants:
+        # the same batch with per-pair FP64 stretching sums
         bt = D.Batch(off, sig, "winckelmans", outputs=cv.OUT_VEL | cv.OUT_DVORT,
-                     accumulate=cv.ACC_TILES)
+                     accumulate=cv.ACC_COMP)
         tt = bt.autotune(dpv, da, v2, d2, reps=1)
         t_step, t_k = tm.run(lambda: bt.eval(dpv, da, v2, d2, stream=stream),
                              max(1, args.steps - 1), 1)
         Vt, Dt = v2.download(), d2.download()
-        tv =np.concatenate([Vt[off[c]:off[c + 1]][i] for c, i in zip(pick, idxs)])
+        tv = np.concatenate([Vt[off[c]:off[c + 1]][i] for c, i in zip(pick, idxs)])
         td = np.concatenate([Dt[off[c]:off[c + 1]][i] for c, i in zip(pick, idxs)])
-        tiles = {"what": "accumulate=CVTX_HIP_ACC_TILES: FP32 tile sums + TwoSum across tiles "
-                         "for every sum, no per-pair FP64 stretching",
+        comp = {"what": "accumulate=CVTX_HIP_ACC_COMP: FP32 tile sums + TwoSum across tiles, "
+                        "FP64 stretching sums per pair",
                  "ms_per_step": t_step, "interactions_per_s": pairs / (t_step * 1e-3),
                  "shape": list(tt[:3]), "roofline": roofline(pairs, F_CLOUDS, t_k),
                  "rel_err_vs_fp64": {"vel": err_stats(tv, rv), "dvort": err_stats(td, rd)}}
-        log(f"  ACC_TILES {t_step:.1f} ms (shape {tt[:3]})")
+        log(f"  ACC_COMP {t_step:.1f} ms (shape {tt[:3]})")
         bt.destroy()
     cpu = None
     if not args.no_cpu:
@@ -470,6 +473,8 @@ def bench_clouds(args, cv, D, W, o, stream):
     line = base_line(args, pairs / (ms_step * 1e-3), "interactions/s", ms_step,
                      {"workload": "512 clouds, sizes log-uniform in [1024, 65536], unit cubes, "
                                   "Winckelmans sigma_c ~ U[0.01, 0.1], fused vel+dvort",
+                      "accumulation": "tile-compensated (CVTX_HIP_ACC_TILES): FP32 tile sums "
+                                      "+ TwoSum across tiles, FP64 epilogue",
                       "n_clouds": n_clouds, "n_particles": n, "pairs_per_eval": pairs,
                       "shape": {"targets_per_lane": tuned[0], "workgroup_size": tuned[1],
                                 "workgroups_per_cu": tuned[2], "tuned_by": "cvtx_hip_batch_autotune",
@@ -490,7 +495,7 @@ def bench_clouds(args, cv, D, W, o, stream):
                  "rk4_graph": rk,
                  "rel_err_vs_fp64": {"samples": int(len(gv)), "clouds": [int(c) for c in pick],
                                      "vel": err_stats(gv, rv), "dvort": err_stats(gd, rd)},
-                 "acc_tiles": tiles,
+                 "acc_comp": comp,
                  "cpu_baseline": cpu})
     b.destroy()
     return line
@@ -875,7 +880,7 @@ def main():
     ap.add_argument("--cpu-samples", type=int, default=512)
     ap.add_argument("--rk4-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variants", default="plain,tiles,visc,single,c_api,fp64final,gaussian")
+    ap.add_argument("--variants", default="plain,acc,visc,single,c_api,fp64final,gaussian")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
