@@ -152,8 +152,8 @@ struct RegKernel<REG_WINCKELMANS> {
     float y2 = y * y;
     float y3 = y2 * y;
     float y5 = y3 * y2;
-    float kv = ((s + 1.5f) * y2) * y3;
-    Kh = ((s + 2.5f) * y2) * y5;
+    float kv = (s + 1.5f) * y5;
+    Kh = (kv + y5) * y2;  // (s + 2.5) y^7 from Kv: two operations instead of three
     float e = (y5 * y2) * y2;
     if constexpr (SELF) {
       bool z = (r2 == 0.f);
