@@ -140,8 +140,18 @@ struct F3DFar {
   static constexpr int N = NP * NR;             // values per shared reciprocal
 };
 
+// Waves per SIMD the register allocation must leave room for.  The fused
+// kernel with one target per lane is held to 5 (at most 96 VGPRs; it takes
+// 103 and 4 waves unbounded, and spills 16 bytes held to 5): 16384
+// filaments x 2^20 targets, 21.29 -> 19.69 ms.  The single-output kernels
+// run best unbounded (vel 10.65 ms at 5 waves against 10.85 held to 7).
+template <int MASK, int T>
+constexpr int f3d_min_waves() {
+  return (MASK == (OUT_VEL | OUT_DVORT) && T == 1) ? 5 : 1;
+}
+
 template <int MASK, int ACC, int T, int WG>
-__global__ __launch_bounds__(WG) void f3d_kernel(F3DKParams p) {
+__global__ __launch_bounds__(WG, (f3d_min_waves<MASK, T>())) void f3d_kernel(F3DKParams p) {
   constexpr int NA = ((MASK & OUT_VEL) ? 3 : 0) + ((MASK & OUT_DVORT) ? 4 : 0);
   constexpr int D = (MASK & OUT_VEL) ? 3 : 0;
   constexpr int WAVES = WG / 64;
@@ -202,9 +212,19 @@ __global__ __launch_bounds__(WG) void f3d_kernel(F3DKParams p) {
         const float rx = b.x - a.x, ry = b.y - a.y, rz = b.z - a.z;
         const float g = a.w * k4pi;
         const float len = __builtin_sqrtf(rx * rx + ry * ry + rz * rz);
+        // link: this filament ends where the next one of the tile starts,
+        // bit for bit (a polyline), so the two can share that point's
+        // distance terms in the far loop
+        int link = 0;
+        if (tid + 1 < cnt) {
+          const float4 an = p.fa[ts + tid + 1];
+          link = (__float_as_uint(an.x) == __float_as_uint(b.x)) &
+                 (__float_as_uint(an.y) == __float_as_uint(b.y)) &
+                 (__float_as_uint(an.z) == __float_as_uint(b.z));
+        }
         s_a[tid] = make_float4(a.x, a.y, a.z, g);
         s_b[tid] = make_float4(b.x, b.y, b.z, 3.f * g / len);
-        s_r0[tid] = make_float4(rx, ry, rz, 0.f);
+        s_r0[tid] = make_float4(rx, ry, rz, __int_as_float(link));
       }
       // the tile's box holds both ends of every filament; extra: the
       // longest filament's squared length
@@ -260,33 +280,68 @@ __global__ __launch_bounds__(WG) void f3d_kernel(F3DKParams p) {
         }
         float cx[FF::NP], cy[FF::NP], cz[FF::NP], ys[FF::NP], y12[FF::NP], p12[FF::NP],
             cn[FF::NP], den[FF::N], inv[FF::N];
+        // the terms of pair q (filament u, target k) from r1 = x - a,
+        // r2 = x - b, their squared norms and reciprocal norms
+        auto terms = [&](int q, int u, float r1x, float r1y, float r1z, float r2x, float r2y,
+                         float r2z, float n1s, float n2s, float y1, float y2) {
+          cx[q] = __builtin_fmaf(r1y, r2z, -r1z * r2y);
+          cy[q] = __builtin_fmaf(r1z, r2x, -r1x * r2z);
+          cz[q] = __builtin_fmaf(r1x, r2y, -r1y * r2x);
+          const float n1 = n1s * y1, n2 = n2s * y2;
+          const float d12 = __builtin_fmaf(r1x, r2x, __builtin_fmaf(r1y, r2y, r1z * r2z));
+          ys[q] = y1 + y2;
+          den[q] = __builtin_fmaf(n1, n2, d12);  // P + r1.r2
+          if constexpr ((MASK & OUT_DVORT) != 0) {
+            y12[q] = y1 * y2;
+            den[FF::NP + q] = n1 + n2;
+            const float cc = __builtin_fmaf(cx[q], cx[q], __builtin_fmaf(cy[q], cy[q], cz[q] * cz[q]));
+            cn[q] = __builtin_amdgcn_sqrtf(cc);
+            p12[q] = __builtin_fmaf(f0[u].x, r1x + r2x,
+                                    __builtin_fmaf(f0[u].y, r1y + r2y, f0[u].z * (r1z + r2z)));
+          }
+        };
+        // a chained group (each filament ends where the next begins) has
+        // U + 1 distinct end points: each point's distance and rsq serve
+        // the two filaments that meet there (same formulas as below).
+        // Lattice of 16384 filaments in chains of 128 x 2^20 targets:
+        // vel 13.22 -> 10.65 ms, dvort 19.23 -> 18.67, fused 20.41 -> 19.69
+        bool chained = false;
+        if constexpr (U > 1) {
+          int f = __float_as_int(f0[0].w);
 #pragma unroll
-        for (int u = 0; u < U; ++u)
+          for (int u = 1; u + 1 < U; ++u) f &= __float_as_int(f0[u].w);
+          chained = __builtin_amdgcn_readfirstlane(f) != 0;
+        }
+        if (chained) {
 #pragma unroll
           for (int k = 0; k < T; ++k) {
-            const int q = u * T + k;
-            const float r1x = xt[k].x - fa[u].x, r1y = xt[k].y - fa[u].y, r1z = xt[k].z - fa[u].z;
-            const float r2x = xt[k].x - fb[u].x, r2y = xt[k].y - fb[u].y, r2z = xt[k].z - fb[u].z;
-            cx[q] = __builtin_fmaf(r1y, r2z, -r1z * r2y);
-            cy[q] = __builtin_fmaf(r1z, r2x, -r1x * r2z);
-            cz[q] = __builtin_fmaf(r1x, r2y, -r1y * r2x);
-            const float n1s = __builtin_fmaf(r1x, r1x, __builtin_fmaf(r1y, r1y, r1z * r1z));
-            const float n2s = __builtin_fmaf(r2x, r2x, __builtin_fmaf(r2y, r2y, r2z * r2z));
-            const float y1 = __builtin_amdgcn_rsqf(n1s);
-            const float y2 = __builtin_amdgcn_rsqf(n2s);
-            const float n1 = n1s * y1, n2 = n2s * y2;
-            const float d12 = __builtin_fmaf(r1x, r2x, __builtin_fmaf(r1y, r2y, r1z * r2z));
-            ys[q] = y1 + y2;
-            den[q] = __builtin_fmaf(n1, n2, d12);  // P + r1.r2
-            if constexpr ((MASK & OUT_DVORT) != 0) {
-              y12[q] = y1 * y2;
-              den[FF::NP + q] = n1 + n2;
-              const float cc = __builtin_fmaf(cx[q], cx[q], __builtin_fmaf(cy[q], cy[q], cz[q] * cz[q]));
-              cn[q] = __builtin_amdgcn_sqrtf(cc);
-              p12[q] = __builtin_fmaf(f0[u].x, r1x + r2x,
-                                      __builtin_fmaf(f0[u].y, r1y + r2y, f0[u].z * (r1z + r2z)));
+            // walk the points: each filament's terms as soon as its end is known
+            float px = xt[k].x - fa[0].x, py = xt[k].y - fa[0].y, pz = xt[k].z - fa[0].z;
+            float pns = __builtin_fmaf(px, px, __builtin_fmaf(py, py, pz * pz));
+            float py1 = __builtin_amdgcn_rsqf(pns);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              const float4 pt = u + 1 < U ? fa[u + 1] : fb[U - 1];
+              const float qx = xt[k].x - pt.x, qy = xt[k].y - pt.y, qz = xt[k].z - pt.z;
+              const float qns = __builtin_fmaf(qx, qx, __builtin_fmaf(qy, qy, qz * qz));
+              const float qy1 = __builtin_amdgcn_rsqf(qns);
+              terms(u * T + k, u, px, py, pz, qx, qy, qz, pns, qns, py1, qy1);
+              px = qx; py = qy; pz = qz; pns = qns; py1 = qy1;
             }
           }
+        } else {
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < T; ++k) {
+              const float r1x = xt[k].x - fa[u].x, r1y = xt[k].y - fa[u].y, r1z = xt[k].z - fa[u].z;
+              const float r2x = xt[k].x - fb[u].x, r2y = xt[k].y - fb[u].y, r2z = xt[k].z - fb[u].z;
+              const float n1s = __builtin_fmaf(r1x, r1x, __builtin_fmaf(r1y, r1y, r1z * r1z));
+              const float n2s = __builtin_fmaf(r2x, r2x, __builtin_fmaf(r2y, r2y, r2z * r2z));
+              terms(u * T + k, u, r1x, r1y, r1z, r2x, r2y, r2z, n1s, n2s,
+                    __builtin_amdgcn_rsqf(n1s), __builtin_amdgcn_rsqf(n2s));
+            }
+        }
         batch_rcp<FF::N>(den, inv);
 #pragma unroll
         for (int u = 0; u < U; ++u)

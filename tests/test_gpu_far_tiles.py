@@ -167,3 +167,43 @@ def test_f3d_targets_on_filament_lines(orc):
         ref = orc.F3D_M2M_dvort(f, t[idx])
         assert np.all(np.isfinite(d))
         assert_parity_limited(f"dvort on lines T={T}", d[idx], ref, d64, TOL)
+
+
+@pytest.mark.parametrize("mask", (1, 2, 3))
+def test_f3d_polylines_partly_chained(orc, mask):
+    # polylines of 1..7 segments: where a filament ends exactly where the
+    # next one begins (bit for bit), the far loop shares that point's
+    # distance terms between the two.  Links break at arbitrary positions,
+    # so groups mix chained and unchained filaments; one link is broken by
+    # a single ulp and must not count as chained.
+    from cvortex_amd.types import make_f3d, make_p3d
+    from gpuutil import assert_parity_limited
+    rng = np.random.default_rng(21)
+    st, en = [], []
+    while len(st) < 3000:
+        m = int(rng.integers(1, 8))
+        path = (rng.uniform(-1, 1, 3) +
+                np.cumsum(rng.normal(scale=0.05, size=(m + 1, 3)), axis=0)).astype(np.float32)
+        st.extend(path[:-1])
+        en.extend(path[1:])
+    st = np.array(st[:3000], np.float32)
+    en = np.array(en[:3000], np.float32)
+    linked = np.all(en[:-1] == st[1:], axis=1)
+    k = int(np.flatnonzero(linked)[5])
+    en[k, 0] = np.nextafter(en[k, 0], np.float32(np.inf))
+    f = make_f3d(st, en, rng.uniform(0.5, 1.5, len(st)))
+    # targets far from every filament (far tiles) and among them (general loop)
+    x = np.concatenate([rng.uniform(3.0, 4.0, size=(12288, 3)),
+                        rng.uniform(-1.2, 1.2, size=(4096, 3))]).astype(np.float32)
+    t = make_p3d(x, rng.normal(size=(len(x), 3)), np.ones(len(x)))
+    idx = np.concatenate([sample(12288, 256, 22), 12288 + sample(4096, 128, 23)])
+    v64, d64 = orc.f64_F3D(f, t, idx=idx.astype(np.int32))
+    for T in (1, 2):
+        case(f"F3D partly chained polylines, mask={mask}, T={T}")
+        v, d = f3d_dev(f, t, mask, (T, 1))
+        if mask & 1:
+            assert_parity_limited(f"vel T={T}", v[idx], orc.F3D_M2M_vel(f, t["coord"][idx]), v64,
+                                  TOL)
+        if mask & 2:
+            assert np.all(np.isfinite(d))
+            assert_parity_limited(f"dvort T={T}", d[idx], orc.F3D_M2M_dvort(f, t[idx]), d64, TOL)
