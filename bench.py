@@ -98,24 +98,30 @@ class Timer:
     def __init__(self, stream):
         self.stream = stream
 
-    def run(self, fn, steps, warmup):
+    def run(self, fn, steps, warmup, per_step=True):
         """(wall ms per step, event ms per step) over `steps` calls after
-        `warmup` untimed ones; self.last_steps_ms holds each step's event
-        time (an event is recorded between consecutive steps)."""
+        `warmup` untimed ones.  per_step: an event is recorded between
+        consecutive steps and self.last_steps_ms holds each step's time;
+        otherwise only around all of them (an event record costs the GPU a
+        few microseconds, which matters for steps of tens of microseconds)."""
         from cvortex_amd import hip as H
         for _ in range(warmup):
             fn()
         H.stream_synchronize(self.stream)
-        ev = [H.Event() for _ in range(steps + 1)]
+        ev = [H.Event() for _ in range((steps if per_step else 1) + 1)]
         w0 = time.perf_counter()
         ev[0].record(self.stream)
         for k in range(steps):
             fn()
-            ev[k + 1].record(self.stream)
+            if per_step:
+                ev[k + 1].record(self.stream)
+        if not per_step:
+            ev[1].record(self.stream)
         ev[-1].synchronize()
         H.stream_synchronize(self.stream)
         wall = time.perf_counter() - w0
-        self.last_steps_ms = [ev[k].elapsed_ms(ev[k + 1]) for k in range(steps)]
+        self.last_steps_ms = ([ev[k].elapsed_ms(ev[k + 1]) for k in range(steps)] if per_step
+                              else [])
         return wall / steps * 1e3, ev[0].elapsed_ms(ev[-1]) / steps
 
 
@@ -284,7 +290,8 @@ def bench_cube(args, cv, D, W, o, stream):
     prob = D.p3d_problem(dev.pv, dev.a, n, dev.pv, None, n, vel, None, None, "winckelmans", sigma,
                          0.0, cv.ACC_COMP)
     steps = max(args.steps, 50)
-    ms_step, ms_k = tm.run(lambda: D.p3d_eval(prob, stream), steps, max(args.warmup, 5))
+    ms_step, ms_k = tm.run(lambda: D.p3d_eval(prob, stream), steps, max(args.warmup, 5),
+                           per_step=False)
     V = vel.download()
     F = FLOPS_PER_PAIR[("p3d", "vel", "winckelmans")]
     # end to end through the public C API (gather, copies, kernels)

@@ -412,6 +412,7 @@ __global__ __launch_bounds__(256) void f3d_reduce(F3DKParams p) {
   double d[NA];
 #pragma unroll
   for (int a = 0; a < NA; ++a) d[a] = 0.0;
+#pragma unroll 4
   for (int s = 0; s < p.n_split; ++s)
 #pragma unroll
     for (int a = 0; a < NA; ++a) d[a] += p.partial[((size_t)s * NA + a) * p.n_tgt + i];

@@ -317,6 +317,7 @@ __global__ __launch_bounds__(256) void p2d_reduce(P2DKParams p) {
   double d[NA];
 #pragma unroll
   for (int a = 0; a < NA; ++a) d[a] = 0.0;
+#pragma unroll 4
   for (int s = 0; s < p.n_split; ++s)
 #pragma unroll
     for (int a = 0; a < NA; ++a) d[a] += p.partial[((size_t)s * NA + a) * p.n_tgt + i];

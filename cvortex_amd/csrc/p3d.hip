@@ -53,6 +53,9 @@ __global__ __launch_bounds__(256) void p3d_reduce(P3DKParams p) {
   double d[NA];
 #pragma unroll
   for (int a = 0; a < NA; ++a) d[a] = 0.0;
+  // partitions added in order; unrolled so that the loads of several
+  // partitions are in flight at once
+#pragma unroll 4
   for (int s = 0; s < p.n_split; ++s)
 #pragma unroll
     for (int a = 0; a < NA; ++a) d[a] += p.partial[((size_t)s * NA + a) * p.n_tgt + i];

@@ -100,8 +100,13 @@ static bool make_ready(Runtime& rt, Device& d) {
   if (!check_hip(rt, hipSetDevice(d.id), "hipSetDevice")) return false;
   if (!check_hip(rt, hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking), "hipStreamCreate"))
     return false;
+  // ordering events: waited on by other streams of this device only, so no
+  // timing and no system-scope fence (a device-scope release suffices)
   for (UseOrder* o : {&d.slot_use, &d.scratch_use})
-    if (!check_hip(rt, hipEventCreateWithFlags(&o->ev, hipEventDisableTiming), "hipEventCreate"))
+    if (!check_hip(rt,
+                   hipEventCreateWithFlags(&o->ev,
+                                           hipEventDisableTiming | hipEventDisableSystemFence),
+                   "hipEventCreate"))
       return false;
   if (!attach_scratch(rt, d)) return false;
   float tab[8 * GTAB_N];

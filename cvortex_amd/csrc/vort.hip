@@ -254,6 +254,7 @@ __global__ __launch_bounds__(256) void vort_reduce(VortKParams p) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= p.n_tgt) return;
   double d[3] = {0.0, 0.0, 0.0};
+#pragma unroll 4
   for (int s = 0; s < p.n_split; ++s)
 #pragma unroll
     for (int a = 0; a < 3; ++a) d[a] += p.partial[((size_t)s * 3 + a) * p.n_tgt + i];
