@@ -289,8 +289,10 @@ def bench_cube(args, cv, D, W, o, stream):
     vel = D.DeviceArray((n, 3))
     prob = D.p3d_problem(dev.pv, dev.a, n, dev.pv, None, n, vel, None, None, "winckelmans", sigma,
                          0.0, cv.ACC_COMP)
-    steps = max(args.steps, 50)
-    ms_step, ms_k = tm.run(lambda: D.p3d_eval(prob, stream), steps, max(args.warmup, 5),
+    # an evaluation is tens of microseconds: at least 200 timed steps after
+    # 20 warm-ups, so the clock has settled and launch jitter averages out
+    steps = max(args.steps, 200)
+    ms_step, ms_k = tm.run(lambda: D.p3d_eval(prob, stream), steps, max(args.warmup, 20),
                            per_step=False)
     V = vel.download()
     F = FLOPS_PER_PAIR[("p3d", "vel", "winckelmans")]
@@ -320,6 +322,17 @@ def bench_cube(args, cv, D, W, o, stream):
                  "rel_err_vs_fp64": {"samples": n, "vel": err_stats(V, v64),
                                      "c_api_vel": err_stats(Ve2e, v64)},
                  "cpu_baseline": cpu})
+    if "shapes" in args.variants:
+        # explicit launch shapes (targets per lane, source partitions):
+        # microseconds per evaluation over 200 back-to-back evaluations
+        by_shape = {}
+        for T in (1, 2, 4):
+            for k in (8, 16, 32):
+                _, kk = tm.run(lambda: D.p3d_eval(prob, stream, shape=(T, k)), 200, 10,
+                               per_step=False)
+                by_shape[f"T={T},partitions={k}"] = kk * 1e3
+        line["us_by_shape"] = by_shape
+        log("  " + ", ".join(f"{k}: {v:.1f} us" for k, v in by_shape.items()))
     return line
 
 

@@ -298,3 +298,38 @@ def test_f3d_p2d_device_shapes(orc):
         assert_parity(f"P2D vel {shape}", v.download(), orc.P2D_M2M_vel(q, q["coord"], "winckelmans", 0.1), TOL)
         assert_parity(f"P2D visc {shape}", e.download(),
                       orc.P2D_M2M_visc_dvort(q, q, "winckelmans", 0.1, 0.05), TOL)
+
+
+def test_tiles_mode_is_the_compensated_mode_for_p2d_f3d_vort():
+    # P2D, F3D and the vorticity field have no per-pair FP64 sums: the
+    # tile-compensated mode runs their compensated kernels, bit for bit
+    case("ACC_TILES == ACC_COMP for P2D, F3D, vorticity field")
+    q = D.DeviceArray.from_host(D.p2d_soa(W.random_p2d(3000, 101, box=1.0, vort=1.0)))
+    outs = []
+    for acc in (cv.ACC_COMP, cv.ACC_TILES):
+        v2 = D.DeviceArray((3000, 2))
+        s2 = D.DeviceArray((3000,))
+        D.p2d_eval(D.P2DProblem(q.addr, 3000, q.addr, 3000, v2.addr, s2.addr,
+                                D.REG_ID["gaussian"], 0.05, 0.01, acc))
+        outs.append((v2.download(), s2.download()))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    fa, fb = D.f3d_soa(W.random_f3d(700, 102, box=1.0, strength=1.0))
+    t = W.random_p3d(2000, 103, box=1.0, vort=1.0)
+    tpv, ta = D.p3d_soa(t)
+    dfa, dfb = D.DeviceArray.from_host(fa), D.DeviceArray.from_host(fb)
+    dtp, dta = D.DeviceArray.from_host(tpv), D.DeviceArray.from_host(ta)
+    outs = []
+    for acc in (cv.ACC_COMP, cv.ACC_TILES):
+        v = D.DeviceArray((2000, 3))
+        dv = D.DeviceArray((2000, 3))
+        D.f3d_eval(D.F3DProblem(dfa.addr, dfb.addr, 700, dtp.addr, dta.addr, 2000, v.addr, dv.addr,
+                                acc))
+        outs.append((v.download(), dv.download()))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    outs = []
+    for acc in (cv.ACC_COMP, cv.ACC_TILES):
+        om = D.DeviceArray((2000, 3))
+        D.vort_eval(D.VortProblem(dtp.addr, dta.addr, 2000, dtp.addr, 2000, om.addr,
+                                  D.REG_ID["gaussian"], 0.1, acc))
+        outs.append(om.download())
+    assert np.array_equal(outs[0], outs[1])
