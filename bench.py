@@ -322,6 +322,18 @@ def bench_cube(args, cv, D, W, o, stream):
                  "rel_err_vs_fp64": {"samples": n, "vel": err_stats(V, v64),
                                      "c_api_vel": err_stats(Ve2e, v64)},
                  "cpu_baseline": cpu})
+    if "plain" in args.variants:
+        # one FP32 running sum per component: the plain summation of the
+        # configuration's statement, next to the compensated default
+        vp = D.DeviceArray((n, 3))
+        prob_p = D.p3d_problem(dev.pv, dev.a, n, dev.pv, None, n, vp, None, None, "winckelmans",
+                               sigma, 0.0, cv.ACC_PLAIN)
+        sp_, kp = tm.run(lambda: D.p3d_eval(prob_p, stream), steps, max(args.warmup, 20),
+                         per_step=False)
+        line["plain_fp32"] = {"interactions_per_s": pairs / (sp_ * 1e-3), "ms_per_step": sp_,
+                              "frac_of_peak": roofline(pairs, F, kp)["frac"],
+                              "rel_err_vs_fp64": err_stats(vp.download(), v64)}
+        log(f"  plain FP32: {sp_ * 1e3:.1f} us")
     if "shapes" in args.variants:
         # explicit launch shapes (targets per lane, source partitions):
         # microseconds per evaluation over 200 back-to-back evaluations

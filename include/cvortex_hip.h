@@ -24,6 +24,9 @@
  *    partials, target ordering) share one grow-only workspace per device:
  *    a call on one stream waits (through an event) for the last such call
  *    on any other stream, so calls on different streams do not overlap.
+ *    That ordering is not recorded into a stream being captured: captured
+ *    calls that need workspace must not run concurrently with each other
+ *    or with eager calls on the same device.
  *    A call captured into a graph keeps the workspace address it saw, and
  *    a later call that needs more workspace replaces that buffer; graphs
  *    should therefore be captured around a size that an earlier eager call
