@@ -102,12 +102,14 @@ static bool make_ready(Runtime& rt, Device& d) {
     return false;
   // ordering events: waited on by other streams of this device only, so no
   // timing and no system-scope fence (a device-scope release suffices)
-  for (UseOrder* o : {&d.slot_use, &d.scratch_use})
+  for (UseOrder* o : {&d.slot_use, &d.scratch_use}) {
     if (!check_hip(rt,
                    hipEventCreateWithFlags(&o->ev,
                                            hipEventDisableTiming | hipEventDisableSystemFence),
                    "hipEventCreate"))
       return false;
+    o->own = d.stream;
+  }
   if (!attach_scratch(rt, d)) return false;
   float tab[8 * GTAB_N];
   build_gaussian_table(tab);
@@ -171,14 +173,27 @@ void* host_buffer(Runtime& rt, Device& d, int slot, size_t bytes) {
 StreamUse::StreamUse(UseOrder& o, hipStream_t s) : o_(o), s_(s) {
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   capturing_ = hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
-  if (!capturing_ && o_.pending && o_.stream != s) (void)hipStreamWaitEvent(s, o_.ev, 0);
+  if (capturing_ || !o_.stream || o_.stream == s) return;
+  if (o_.lazy) {
+    // the last use ran on the device's own stream: mark it now
+    o_.pending = hipEventRecord(o_.ev, o_.stream) == hipSuccess;
+    o_.lazy = false;
+  }
+  if (o_.pending) (void)hipStreamWaitEvent(s, o_.ev, 0);
 }
 
 StreamUse::~StreamUse() {
   if (capturing_ || !o_.ev) return;
+  if (s_ == o_.own) {
+    o_.stream = s_;
+    o_.lazy = true;
+    o_.pending = false;
+    return;
+  }
   if (hipEventRecord(o_.ev, s_) == hipSuccess) {
     o_.stream = s_;
     o_.pending = true;
+    o_.lazy = false;
   }
 }
 

@@ -46,12 +46,16 @@ struct Buffer {
   size_t bytes = 0;
 };
 
-// The last use of a shared device buffer: an event recorded on `stream`
-// after that use (pending = recorded and not yet superseded).
+// The last use of a shared device buffer, on `stream`: either an event
+// recorded after it (pending), or, when that stream is the device's own
+// (`own`, which lives as long as the device), nothing yet (lazy): the event
+// is recorded on `own` only when a call on another stream needs to wait.
 struct UseOrder {
   hipEvent_t ev = nullptr;
   hipStream_t stream = nullptr;
+  hipStream_t own = nullptr;
   bool pending = false;
+  bool lazy = false;
 };
 
 struct Device {
@@ -106,9 +110,11 @@ void set_error(Runtime& rt, const std::string& msg);
 // Orders the calls that use a shared device buffer across streams: made at
 // the start of a call on stream s, it makes s wait for the last such call
 // if that ran on another stream; at its end it marks this call's work on s.
-// Nothing is recorded or waited for while s is being captured into a graph
-// (the graph orders its own nodes).  StreamUse(d, s) guards the slot
-// buffers.
+// A call on the device's own stream records nothing (an event record costs
+// the GPU a few microseconds between launches); its work is marked when,
+// and if, a call on another stream comes.  Nothing is recorded or waited
+// for while s is being captured into a graph (the graph orders its own
+// nodes).  StreamUse(d, s) guards the slot buffers.
 class StreamUse {
  public:
   StreamUse(UseOrder& o, hipStream_t s);

@@ -112,5 +112,28 @@ class Event:
             pass
 
 
+class Stream:
+    """A non-blocking HIP stream at the default priority."""
+
+    def __init__(self):
+        self.h = ctypes.c_void_p()
+        check(hip().hipStreamCreateWithFlags(ctypes.byref(self.h), ctypes.c_uint(1)),
+              "hipStreamCreateWithFlags")
+
+    @property
+    def addr(self):
+        return self.h.value
+
+    def synchronize(self):
+        check(hip().hipStreamSynchronize(self.h), "hipStreamSynchronize")
+
+    def __del__(self):
+        try:
+            hip().hipStreamSynchronize(self.h)
+            hip().hipStreamDestroy(self.h)
+        except Exception:
+            pass
+
+
 def stream_synchronize(stream):
     check(hip().hipStreamSynchronize(ctypes.c_void_p(stream)), "hipStreamSynchronize")

@@ -333,3 +333,37 @@ def test_tiles_mode_is_the_compensated_mode_for_p2d_f3d_vort():
                                   D.REG_ID["gaussian"], 0.1, acc))
         outs.append(om.download())
     assert np.array_equal(outs[0], outs[1])
+
+
+def test_workspace_calls_on_several_streams():
+    # device-resident calls on different streams share the device's
+    # workspace (the FP64 partials of source partitions, the target
+    # permutation): the library orders them, so calls queued on two caller
+    # streams and the library's own stream, with no synchronisation between
+    # them, give the results of one stream bit for bit
+    from cvortex_amd import hip as H
+    case("workspace ordering across streams")
+    n = 20000
+    p = W.random_p3d(n, 111, box=1.0, vort=1.0)
+    dev = D.P3DDevice(p)
+    ref_outs = [dev.outputs(True, True, False) for _ in range(2)]
+    probs = [dev.problem(ref_outs[0], "winckelmans", 0.1),
+             dev.problem(ref_outs[1], "gaussian", 0.05)]
+    for pr in probs:  # the workspace reaches its size on the library stream
+        D.p3d_eval(pr, None, shape=(1, 16))
+    D.synchronize()
+    ref = [[o.download() for o in outs[:2]] for outs in ref_outs]
+    a, b = H.Stream(), H.Stream()
+    for rep in range(4):
+        outs = [dev.outputs(True, True, False) for _ in range(3)]
+        p0 = dev.problem(outs[0], "winckelmans", 0.1)
+        p1 = dev.problem(outs[1], "gaussian", 0.05)
+        p2 = dev.problem(outs[2], "winckelmans", 0.1)
+        D.p3d_eval(p0, a.addr, shape=(1, 16))
+        D.p3d_eval(p1, b.addr, shape=(1, 16))
+        D.p3d_eval(p2, None, shape=(1, 16))
+        D.p3d_eval(p1, b.addr, shape=(1, 16))
+        D.synchronize()
+        for got, want in ((outs[0], ref[0]), (outs[1], ref[1]), (outs[2], ref[0])):
+            assert np.array_equal(got[0].download(), want[0])
+            assert np.array_equal(got[1].download(), want[1])
