@@ -70,6 +70,16 @@ def test_ring_2pow20(orc):
     v2, d2, _ = cv.P3D_M2M_fused(p, p, "gaussian", -sigma, 0.0, vel=True, dvort=True)
     assert np.array_equal(v2, v)
     assert np.array_equal(d2, -d)
+    case("ring: tile-compensated sums (no per-pair FP64) within the contract")
+    prev = cv.set_accumulation(cv.ACC_TILES)
+    try:
+        vt, dt, _ = cv.P3D_M2M_fused(p, p, "gaussian", sigma, 0.0, vel=True, dvort=True)
+    finally:
+        cv.set_accumulation(prev)
+    for name, g, r in (("vel", vt[idx], v64), ("dvort", dt[idx], d64)):
+        e, rms = oracle.rel_l2(g, r), rms_rel(g, r)
+        print(f"  tiles {name}: relL2 vs FP64 {e:.2e}, RMS per target {rms:.2e}", flush=True)
+        assert e <= TOL
 
 
 def test_cube_8192(orc):
