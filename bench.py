@@ -323,8 +323,8 @@ def bench_cube(args, cv, D, W, o, stream):
                                      "c_api_vel": err_stats(Ve2e, v64)},
                  "cpu_baseline": cpu})
     if "plain" in args.variants:
-        # one FP32 running sum per component: the plain summation of the
-        # configuration's statement, next to the compensated default
+        # one FP32 running sum per component (plain summation), next to the
+        # compensated default
         vp = D.DeviceArray((n, 3))
         prob_p = D.p3d_problem(dev.pv, dev.a, n, dev.pv, None, n, vp, None, None, "winckelmans",
                                sigma, 0.0, cv.ACC_PLAIN)
