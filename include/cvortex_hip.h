@@ -324,7 +324,10 @@ int cvtx_hip_batch_shape(const cvtx_hip_batch* batch, int* targets_per_lane, int
  * batch's signature (regularisation, outputs, accumulation, log2 of the
  * pair count, log2 of the cloud count): batches created later with the same
  * signature start in that shape, and cvtx_hip_tune_save/load persist it.
- * Must precede cvtx_hip_batch_rk4_init.  Returns 0 and the winner, or -1. */
+ * Must precede cvtx_hip_batch_rk4_init.  Runs on the library's stream and
+ * returns when the timing is done: work of the caller's own streams that
+ * writes pv or a must be complete before the call.  Returns 0 and the
+ * winner, or -1. */
 int cvtx_hip_batch_autotune(cvtx_hip_batch* batch, const float* pv, const float* a, float* vel,
                             float* dvort, float* visc, int reps, int* targets_per_lane,
                             int* workgroup_size, int* workgroups_per_cu, float* best_ms);
