@@ -112,7 +112,12 @@ LaunchCfg choose_cfg(int kind, int reg, int mask, int acc, int n_src, int n_tgt,
     const long long few = (64LL * cus + blocks - 1) / blocks;
     const long long per = ((long long)n_src + 65535) / 65536;
     const long long many = (256LL * cus + blocks - 1) / blocks;
-    const long long k = T == 4 ? many : std::min(std::max(few, per), many);
+    long long k = T == 4 ? many : std::min(std::max(few, per), many);
+    // partitions of one tile spend much of a workgroup on its prologue and
+    // epilogue: two tiles each while that still leaves 32 workgroups per CU
+    // (ring 32768: 128 partitions 0.85 ms, 64 partitions 0.77 ms)
+    const long long tiles = ((long long)n_src + 255) / 256;
+    if (k > tiles / 2 && blocks * (tiles / 2) >= 32LL * cus) k = tiles / 2;
     return launch_shape(T, (int)std::min<long long>(k, 1 << 20), n_src);
   }
   LaunchCfg c = default_cfg(n_src, n_tgt, n_cu);
