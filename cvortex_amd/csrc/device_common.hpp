@@ -23,10 +23,10 @@ namespace cvtx {
 enum Reg : int { REG_SINGULAR = 0, REG_WINCKELMANS = 1, REG_PLANETARY = 2, REG_GAUSSIAN = 3 };
 enum OutMask : int { OUT_VEL = 1, OUT_DVORT = 2, OUT_VISC = 4 };
 // ACC_PLAIN: one FP32 running sum per accumulator.  ACC_COMP: FP32 tile
-// sums folded by TwoSum, and the P3D stretching (and PSE) sums per pair in
-// FP64.  ACC_TILES: FP32 tile sums folded by TwoSum for every accumulator,
-// no per-pair FP64 (P2D, F3D and the vorticity field have no per-pair FP64,
-// so there it is ACC_COMP).
+// sums folded by TwoSum, and the P3D stretching (and PSE) sums and the P2D
+// viscous exchange per pair in FP64.  ACC_TILES: FP32 tile sums folded by
+// TwoSum for every accumulator, no per-pair FP64 (F3D and the vorticity
+// field have no per-pair FP64, so there it is ACC_COMP).
 enum AccMode : int { ACC_PLAIN = 0, ACC_COMP = 1, ACC_TILES = 2 };
 inline bool valid_acc(int a) { return a == ACC_PLAIN || a == ACC_COMP || a == ACC_TILES; }
 

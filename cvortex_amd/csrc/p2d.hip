@@ -6,8 +6,13 @@
 //
 // Reduced sums per target (d = x_t - x_s, q = r^2 / sigma^2):
 //   vel  = (1/2pi) * scale_K * ( sum K a_s d_y, -sum K a_s d_x )
-//   visc = (2 nu / sigma^2) * scale_eta * ( A_t sum eta a_s - a_t sum eta A_s )
-// with K = g_2D(rho)/r^2 in the units below.  Reference semantics:
+//   visc = (2 nu / sigma^2) * scale_eta * sum eta (A_t a_s - a_t A_s)
+// with K = g_2D(rho)/r^2 in the units below.  The exchange is summed pair by
+// pair, as the reference does: its two products nearly cancel for many
+// targets, and two separate sums (A_t sum eta a_s and a_t sum eta A_s)
+// would carry the rounding of the larger sums into the small difference.
+// ACC_COMP sums it in FP64 (each pair's difference of two float products is
+// exact there up to one rounding); ACC_TILES and ACC_PLAIN in FP32.  Reference semantics:
 // src/P2D.cpp:49-69 (vel pair), :99-162 (M2M), :167-276 (visc);
 // src/VortFunc.cpp:78-80,117-131,150-153,191-199 (g_2D, eta_2D).
 #include <hip/hip_runtime.h>
@@ -19,7 +24,7 @@
 
 namespace cvtx {
 
-int p2d_nacc(int mask) { return ((mask & OUT_VEL) ? 2 : 0) + ((mask & OUT_VISC) ? 2 : 0); }
+int p2d_nacc(int mask) { return ((mask & OUT_VEL) ? 2 : 0) + ((mask & OUT_VISC) ? 1 : 0); }
 
 template <int REG>
 struct Reg2D;
@@ -116,7 +121,7 @@ struct P2DKParams {
 };
 
 template <int MASK>
-__device__ __forceinline__ void p2d_finish(const double* a, float4 t, const P2DKParams& p, int i) {
+__device__ __forceinline__ void p2d_finish(const double* a, const P2DKParams& p, int i) {
   constexpr int V = 0;
   constexpr int E = (MASK & OUT_VEL) ? 2 : 0;
   if constexpr ((MASK & OUT_VEL) != 0) {
@@ -124,7 +129,7 @@ __device__ __forceinline__ void p2d_finish(const double* a, float4 t, const P2DK
     p.out_vel[2 * (size_t)i + 1] = (float)(-p.sc_vel * a[V + 1]);
   }
   if constexpr ((MASK & OUT_VISC) != 0) {
-    p.out_visc[i] = (float)(p.sc_visc * ((double)t.w * a[E + 0] - (double)t.z * a[E + 1]));
+    p.out_visc[i] = (float)(p.sc_visc * a[E]);
   }
 }
 
@@ -148,9 +153,11 @@ struct P2DFar {
 
 template <int REG, int MASK, int ACC, int T, int WG>
 __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
-  constexpr int NA = ((MASK & OUT_VEL) ? 2 : 0) + ((MASK & OUT_VISC) ? 2 : 0);
+  constexpr int NA = ((MASK & OUT_VEL) ? 2 : 0) + ((MASK & OUT_VISC) ? 1 : 0);
   constexpr int E = (MASK & OUT_VEL) ? 2 : 0;
   constexpr bool CULL = P2DFar<REG>::ENABLED && (MASK & OUT_VEL) != 0;
+  constexpr bool TS = ACC != ACC_PLAIN;                           // tile sums folded by TwoSum
+  constexpr bool V64 = ACC == ACC_COMP && (MASK & OUT_VISC) != 0;  // exchange summed in FP64
   constexpr int WAVES = WG / 64;
   __shared__ float4 s_src[WG];
   __shared__ BoxPartials<2, WAVES> s_box;
@@ -168,6 +175,13 @@ __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
   for (int k = 0; k < T; ++k)
 #pragma unroll
     for (int a = 0; a < NA; ++a) { acc[k][a] = 0.f; hi[k][a] = 0.f; lo[k][a] = 0.f; }
+  double vacc[T], tA[T], ta[T];
+#pragma unroll
+  for (int k = 0; k < T; ++k) {
+    vacc[k] = 0.0;
+    tA[k] = (double)xt[k].w;
+    ta[k] = (double)xt[k].z;
+  }
 
   // box of the block's targets (the clamped duplicates are real targets)
   Box<2> tbox;
@@ -200,7 +214,7 @@ __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
       box_partial<2, WAVES>(x2, tid < cnt, s_box);
     }
     __syncthreads();
-    if constexpr (ACC == ACC_COMP) {
+    if constexpr (TS) {
 #pragma unroll
       for (int k = 0; k < T; ++k)
 #pragma unroll
@@ -277,18 +291,21 @@ __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
             acc[k][0] = __builtin_fmaf(ka, dy, acc[k][0]);
             acc[k][1] = __builtin_fmaf(ka, dx, acc[k][1]);
           }
-          if constexpr ((MASK & OUT_VISC) != 0) {
-            acc[k][E + 0] = __builtin_fmaf(eta, s.z, acc[k][E + 0]);
-            acc[k][E + 1] = __builtin_fmaf(eta, s.w, acc[k][E + 1]);
+          if constexpr (V64) {
+            const double d = __builtin_fma(tA[k], (double)s.z, -(ta[k] * (double)s.w));
+            vacc[k] = __builtin_fma(d, (double)eta, vacc[k]);
+          } else if constexpr ((MASK & OUT_VISC) != 0) {
+            const float d = __builtin_fmaf(xt[k].w, s.z, -(xt[k].z * s.w));
+            acc[k][E] = __builtin_fmaf(d, eta, acc[k][E]);
           }
         }
       }
     }
-    if constexpr (ACC == ACC_COMP) {
+    if constexpr (TS) {
 #pragma unroll
       for (int k = 0; k < T; ++k)
 #pragma unroll
-        for (int a = 0; a < NA; ++a) two_sum_add(hi[k][a], lo[k][a], acc[k][a]);
+        for (int a = 0; a < (V64 ? E : NA); ++a) two_sum_add(hi[k][a], lo[k][a], acc[k][a]);
     }
   }
 
@@ -299,9 +316,10 @@ __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
     double d[NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a)
-      d[a] = (ACC == ACC_COMP) ? (double)hi[k][a] + (double)lo[k][a] : (double)acc[k][a];
+      d[a] = TS ? (double)hi[k][a] + (double)lo[k][a] : (double)acc[k][a];
+    if constexpr (V64) d[E] = vacc[k];
     if (p.n_split == 1) {
-      p2d_finish<MASK>(d, xt[k], p, i);
+      p2d_finish<MASK>(d, p, i);
     } else {
 #pragma unroll
       for (int a = 0; a < NA; ++a) p.partial[((size_t)blockIdx.y * NA + a) * p.n_tgt + i] = d[a];
@@ -311,7 +329,7 @@ __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
 
 template <int MASK>
 __global__ __launch_bounds__(256) void p2d_reduce(P2DKParams p) {
-  constexpr int NA = ((MASK & OUT_VEL) ? 2 : 0) + ((MASK & OUT_VISC) ? 2 : 0);
+  constexpr int NA = ((MASK & OUT_VEL) ? 2 : 0) + ((MASK & OUT_VISC) ? 1 : 0);
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= p.n_tgt) return;
   double d[NA];
@@ -321,7 +339,7 @@ __global__ __launch_bounds__(256) void p2d_reduce(P2DKParams p) {
   for (int s = 0; s < p.n_split; ++s)
 #pragma unroll
     for (int a = 0; a < NA; ++a) d[a] += p.partial[((size_t)s * NA + a) * p.n_tgt + i];
-  p2d_finish<MASK>(d, p.tgt[i], p, i);
+  p2d_finish<MASK>(d, p, i);
 }
 
 bool p2d_supported(int reg, int mask) {
@@ -347,9 +365,13 @@ K2Fn p2_T(int T) {
     default: return p2d_kernel<REG, MASK, ACC, 2, 256>;
   }
 }
+// ACC_TILES differs from ACC_COMP only in the viscous exchange's sum
 template <int REG, int MASK>
 K2Fn p2_acc(int acc, int T) {
-  return acc == ACC_PLAIN ? p2_T<REG, MASK, ACC_PLAIN>(T) : p2_T<REG, MASK, ACC_COMP>(T);
+  if (acc == ACC_PLAIN) return p2_T<REG, MASK, ACC_PLAIN>(T);
+  if constexpr ((MASK & OUT_VISC) != 0)
+    if (acc == ACC_TILES) return p2_T<REG, MASK, ACC_TILES>(T);
+  return p2_T<REG, MASK, ACC_COMP>(T);
 }
 template <int REG>
 K2Fn p2_mask_inviscid(int mask, int acc, int T) {

@@ -62,11 +62,14 @@ extern "C" {
 /* FP32 tile sums + TwoSum across tiles, FP64 across partitions; P3D also sums
  * the stretching (and PSE) terms of near pairs in FP64 per pair: coherent
  * fields (a vortex ring) need it, their near terms being large and nearly
- * parallel to the target's strength */
+ * parallel to the target's strength; P2D sums its viscous exchange
+ * (A_t a_s - a_t A_s) eta in FP64 per pair, the two products nearly
+ * cancelling for many targets */
 #define CVTX_HIP_ACC_COMP 1
 /* FP32 tile sums + TwoSum across tiles for every sum, no per-pair FP64:
  * enough for fields without that coherence (independent random strengths);
- * the same as CVTX_HIP_ACC_COMP for P2D, F3D and the vorticity field */
+ * the same as CVTX_HIP_ACC_COMP for P2D velocity, F3D and the vorticity
+ * field */
 #define CVTX_HIP_ACC_TILES 2
 
 /* ---- 1. seam: replaces opencl_brute_force_* (reference src/ocl_P3D.h:33-68,

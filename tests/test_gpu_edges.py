@@ -301,9 +301,10 @@ def test_f3d_p2d_device_shapes(orc):
 
 
 def test_tiles_mode_is_the_compensated_mode_for_p2d_f3d_vort():
-    # P2D, F3D and the vorticity field have no per-pair FP64 sums: the
-    # tile-compensated mode runs their compensated kernels, bit for bit
-    case("ACC_TILES == ACC_COMP for P2D, F3D, vorticity field")
+    # P2D velocity, F3D and the vorticity field have no per-pair FP64 sums:
+    # the tile-compensated mode runs their compensated kernels, bit for bit;
+    # the P2D viscous exchange is summed in FP32 tiles instead of FP64 pairs
+    case("ACC_TILES == ACC_COMP for P2D vel, F3D, vorticity field")
     q = D.DeviceArray.from_host(D.p2d_soa(W.random_p2d(3000, 101, box=1.0, vort=1.0)))
     outs = []
     for acc in (cv.ACC_COMP, cv.ACC_TILES):
@@ -312,7 +313,8 @@ def test_tiles_mode_is_the_compensated_mode_for_p2d_f3d_vort():
         D.p2d_eval(D.P2DProblem(q.addr, 3000, q.addr, 3000, v2.addr, s2.addr,
                                 D.REG_ID["gaussian"], 0.05, 0.01, acc))
         outs.append((v2.download(), s2.download()))
-    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert oracle.rel_l2(outs[1][1], outs[0][1]) <= 1e-5
     fa, fb = D.f3d_soa(W.random_f3d(700, 102, box=1.0, strength=1.0))
     t = W.random_p3d(2000, 103, box=1.0, vort=1.0)
     tpv, ta = D.p3d_soa(t)

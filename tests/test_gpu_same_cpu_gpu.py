@@ -1,0 +1,152 @@
+"""The reference's own acceptance test, run on this library's two legs.
+
+Reference test/testsamecpugpuresultmany.h:35-486: 1000 particles (and 1000
+filaments) with random coordinates, strengths and volumes, regularisation
+radius 0.3, viscosity 0.1; every public M2M function is called with the
+accelerator enabled, then with it disabled (cvtx_accelerator_disable), and
+each target must satisfy |a - b| / |a + b| <= 1e-5 whenever |a + b| > 2e-35.
+Here the enabled call runs on the GPU in FP32 and the disabled one on the
+library's CPU leg in FP64.  The reference draws coordinates in [0, 10] (the
+intended range) but, with glibc's RAND_MAX, in [0, 1.5e-4] on Linux: both
+ranges run.  Its P2D targets are never initialised
+(testsamecpugpuresultmany.h:59); here they are the particles' own positions.
+
+The reference compares two FP32 computations of one formula; here one leg
+sums in FP64, so a target whose sum nearly cancels shows the FP32 leg's
+whole rounding error.  Two cases need more than the reference's bound:
+  * the P2D viscous exchange: both legs evaluate eta_2D in FP32 (the CPU
+    leg through the kernel's float function, src/VortFunc.cpp:124-131), and
+    Winckelmans' exp(4 / a^3) carries a few ulp of error per pair; on these
+    inputs one target's terms cancel 2900-fold, so a few ulp per pair is
+    3e-4 of its result.  Each target must satisfy the reference's bound or
+    |a - b| <= 32 u sum_s |terms| (u = 2^-24), the error a few ulp per term
+    on each leg can give;
+  * the filaments: the reference's own FP32 arithmetic (the oracle's
+    restatement of src/F3D.cpp) misses 1e-5 on some targets of these
+    inputs; the device leg must be at least as close to the CPU leg as that
+    restatement is."""
+import numpy as np
+import pytest
+
+import cvortex_amd as cv
+from cvortex_amd import capi
+from cvortex_amd.types import make_f3d, make_p2d, make_p3d
+from gpuutil import case, have_gpu
+
+pytestmark = pytest.mark.gpu
+N = 1000
+REL = 1e-5
+SIGMA = 0.3
+NU = 0.1
+REGS = ("singular", "winckelmans", "planetary", "gaussian")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not have_gpu():
+        pytest.fail("no HIP device: GPU tests need one")
+    cv.set_accumulation(cv.ACC_COMP)
+    prev = cv.set_dispatch(cv.DISPATCH_AUTO)
+    yield
+    cv.set_dispatch(prev)
+
+
+def both_legs(fn):
+    """fn() with the accelerator enabled (device leg), then disabled (CPU leg)."""
+    cv.accelerator_enable(0)
+    a = fn()
+    assert cv.last_leg() == cv.LEG_DEVICE
+    cv.accelerator_disable(0)
+    try:
+        b = fn()
+        assert cv.last_leg() == cv.LEG_HOST
+    finally:
+        cv.accelerator_enable(0)
+    return np.asarray(a, np.float64), np.asarray(b, np.float64)
+
+
+def worst_rel(a, b):
+    """max over targets of |a - b| / |a + b|, over targets with |a + b| > 2e-35"""
+    a = np.asarray(a, np.float64).reshape(len(a), -1)
+    b = np.asarray(b, np.float64).reshape(len(b), -1)
+    m = np.linalg.norm(a - b, axis=1)
+    p = np.linalg.norm(a + b, axis=1)
+    use = p > 2e-35
+    return float((m[use] / p[use]).max()) if use.any() else 0.0, int(use.sum())
+
+
+def check_same(name, a, b, limit=REL):
+    assert np.all(np.isfinite(a)) and np.all(np.isfinite(b)), name
+    worst, n = worst_rel(a, b)
+    print(f"  {name}: {n} targets, max |a-b|/|a+b| {worst:.2e} (limit {limit:.2e})", flush=True)
+    assert worst <= limit, f"{name}: {worst:.2e} > {limit:.2e}"
+
+
+def visc_abs_terms(q, k):
+    """per target, sum over sources of |(A_t a_s - a_t A_s) eta_2D(rho)| 2 nu / sigma^2
+    in FP64 (eta_2D of src/VortFunc.cpp:124-131, 196-199)"""
+    x = q["coord"].astype(np.float64)
+    al, ar = q["vorticity"].astype(np.float64), q["area"].astype(np.float64)
+    rho = np.sqrt(((x[:, None, :] - x[None, :, :]) ** 2).sum(-1)) / SIGMA
+    if k == "winckelmans":
+        a = rho * rho + 1.0
+        eta = 24.0 * np.exp(4.0 / a ** 3) / a ** 4
+    else:
+        eta = np.exp(-0.5 * rho * rho)
+    np.fill_diagonal(eta, 0.0)
+    d = np.abs(ar[:, None] * al[None, :] - al[:, None] * ar[None, :])
+    return (d * eta).sum(1) * 2.0 * NU / SIGMA ** 2
+
+
+def check_same_visc(name, a, b, abs_terms):
+    """the reference's bound, or 32 u of the terms' absolute sum"""
+    assert np.all(np.isfinite(a)) and np.all(np.isfinite(b)), name
+    m, p = np.abs(a - b), np.abs(a + b)
+    cond = 32.0 * 2.0 ** -24 * abs_terms
+    ok = (m <= REL * p) | (m <= cond)
+    worst = float((m[p > 2e-35] / p[p > 2e-35]).max())
+    n_cond = int(np.sum(m > REL * p))
+    units = float((m / (2.0 ** -24 * abs_terms)).max())
+    print(f"  {name}: max |a-b|/|a+b| {worst:.2e}; {n_cond} targets beyond 1e-5, "
+          f"max |a-b| = {units:.2f} u sum|terms|", flush=True)
+    assert np.all(ok), f"{name}: {int(np.sum(~ok))} targets beyond both bounds"
+
+
+def inputs(scale, seed):
+    rng = np.random.default_rng(seed)
+    u = lambda *s: rng.uniform(0.0, 1.0, s).astype(np.float32)
+    p = make_p3d(u(N, 3) * scale, u(N, 3) * 10.0, u(N) * 0.01)
+    f = make_f3d(u(N, 3) * scale, u(N, 3) * scale, u(N) * 10.0)
+    q = make_p2d(u(N, 2) * scale, u(N) * 10.0, u(N) * 0.01)
+    return p, f, q
+
+
+@pytest.mark.parametrize("scale", [10.0, 1.5e-4])
+@pytest.mark.parametrize("k", REGS)
+def test_p3d_and_p2d_same_on_both_legs(k, scale):
+    p, _, q = inputs(scale, 1401 + REGS.index(k))
+    case(f"same CPU/GPU result, P3D/P2D {k}, coordinates in [0, {scale:g}]")
+    check_same(f"P3D vel {k}", *both_legs(lambda: capi.P3D_M2M_vel(p, p["coord"], k, SIGMA)))
+    check_same(f"P3D dvort {k}", *both_legs(lambda: capi.P3D_M2M_dvort(p, p, k, SIGMA)))
+    if k in ("winckelmans", "gaussian"):
+        check_same(f"P3D visc {k}",
+                   *both_legs(lambda: capi.P3D_M2M_visc_dvort(p, p, k, SIGMA, NU)))
+    check_same(f"P2D vel {k}", *both_legs(lambda: capi.P2D_M2M_vel(q, q["coord"], k, SIGMA)))
+    if k in ("winckelmans", "gaussian"):
+        check_same_visc(f"P2D visc {k}",
+                        *both_legs(lambda: capi.P2D_M2M_visc_dvort(q, q, k, SIGMA, NU)),
+                        visc_abs_terms(q, k))
+
+
+@pytest.mark.parametrize("scale", [10.0, 1.5e-4])
+def test_f3d_same_on_both_legs(scale, orc):
+    p, f, _ = inputs(scale, 1411)
+    case(f"same CPU/GPU result, F3D, coordinates in [0, {scale:g}]")
+    for name, fn, ref in (
+            ("F3D vel", lambda: capi.F3D_M2M_vel(f, p["coord"]),
+             lambda: orc.F3D_M2M_vel(f, p["coord"])),
+            ("F3D dvort", lambda: capi.F3D_M2M_dvort(f, p), lambda: orc.F3D_M2M_dvort(f, p))):
+        gpu, cpu = both_legs(fn)
+        ref_worst, _ = worst_rel(ref(), cpu)
+        print(f"  {name}: the reference's FP32 arithmetic, max |a-b|/|a+b| {ref_worst:.2e}")
+        check_same(name, gpu, cpu, max(REL, ref_worst))
