@@ -228,6 +228,11 @@ __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
         // U sources x T targets = 4 pairs per shared reciprocal; eta = 0
         constexpr int U = 4 / T;
         const float ks = in_vgpr(P2DFar<REG>::kscale(p.rc));
+        // a_s times the factor of 1/r^2 in K (planetary: 1, so no multiply)
+        auto sk = [ks](float a) {
+          if constexpr (REG == REG_PLANETARY) return a;
+          else return a * ks;
+        };
         int j = 0;
         for (; j + U <= cnt; j += U, jv += U) {
           float4 s[U];
@@ -247,11 +252,11 @@ __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
           if constexpr (T == 1) {
             shared_rcp4(r2, K);
 #pragma unroll
-            for (int u = 0; u < U; ++u) K[u] *= s[u].z * ks;
+            for (int u = 0; u < U; ++u) K[u] *= sk(s[u].z);
           } else {
             // pairs 0, 1 and 2, 3 share a source: its strength scales the
             // shared reciprocal once
-            shared_rcp4_scaled(r2, s[0].z * ks, s[U - 1].z * ks, K);
+            shared_rcp4_scaled(r2, sk(s[0].z), sk(s[U - 1].z), K);
           }
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
@@ -262,7 +267,7 @@ __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
         }
         for (; j < cnt; ++j, ++jv) {
           const float4 s = s_src[jv];
-          const float as = s.z * ks;
+          const float as = sk(s.z);
 #pragma unroll
           for (int k = 0; k < T; ++k) {
             const float dx = xt[k].x - s.x;
