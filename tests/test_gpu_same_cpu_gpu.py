@@ -1,7 +1,8 @@
-"""The reference's own acceptance test, run on this library's two legs.
+"""The reference's own acceptance tests, run on this library's two legs.
 
-Reference test/testsamecpugpuresultmany.h:35-486: 1000 particles (and 1000
-filaments) with random coordinates, strengths and volumes, regularisation
+Reference test/testsamecpugpuresultmany.h:35-486 (and, with only one
+vortical source, test/testsamecpugpuresultsingle.h:35-461): 1000 particles
+(and 1000 filaments) with random coordinates, strengths and volumes, regularisation
 radius 0.3, viscosity 0.1; every public M2M function is called with the
 accelerator enabled, then with it disabled (cvtx_accelerator_disable), and
 each target must satisfy |a - b| / |a + b| <= 1e-5 whenever |a + b| > 2e-35.
@@ -146,6 +147,56 @@ def test_f3d_same_on_both_legs(scale, orc):
             ("F3D vel", lambda: capi.F3D_M2M_vel(f, p["coord"]),
              lambda: orc.F3D_M2M_vel(f, p["coord"])),
             ("F3D dvort", lambda: capi.F3D_M2M_dvort(f, p), lambda: orc.F3D_M2M_dvort(f, p))):
+        gpu, cpu = both_legs(fn)
+        ref_worst, _ = worst_rel(ref(), cpu)
+        print(f"  {name}: the reference's FP32 arithmetic, max |a-b|/|a+b| {ref_worst:.2e}")
+        check_same(name, gpu, cpu, max(REL, ref_worst))
+
+
+def single_inputs(scale, seed):
+    """test/testsamecpugpuresultsingle.h:73-137: only the first particle,
+    filament and 2D particle carry vorticity or circulation; the targets
+    of dvort, visc and the filaments' dvort carry their own strengths"""
+    rng = np.random.default_rng(seed)
+    u = lambda *s: rng.uniform(0.0, 1.0, s).astype(np.float32)
+    x3 = u(N, 3) * scale
+    a3 = np.zeros((N, 3), np.float32)
+    a3[0] = (u(3) - 0.5) * 10.0
+    vol = u(N) * 0.01
+    p = make_p3d(x3, a3, vol)
+    t = make_p3d(x3, np.vstack([a3[:1], (u(N - 1, 3) - 0.5) * 10.0]), vol)
+    start, end = u(N, 3) * scale, u(N, 3) * scale
+    end[1:] += 0.01 * start[1:]
+    g = np.zeros(N, np.float32)
+    g[0] = (u() - 0.5) * 10.0
+    f = make_f3d(start, end, g)
+    x2 = u(N, 2) * scale
+    w2 = np.zeros(N, np.float32)
+    w2[0] = (u() - 0.5) * 10.0
+    area = u(N) * 0.01
+    q = make_p2d(x2, w2, area)
+    tq = make_p2d(x2, np.concatenate([w2[:1], u(N - 1) * 10.0]), area)
+    return p, t, f, q, tq
+
+
+@pytest.mark.parametrize("scale", [10.0, 1.5e-4])
+def test_single_vortical_source_same_on_both_legs(scale, orc):
+    p, t, f, q, tq = single_inputs(scale, 1421)
+    case(f"same CPU/GPU result, one vortical source, coordinates in [0, {scale:g}]")
+    for k in REGS:
+        check_same(f"P3D vel {k}", *both_legs(lambda: capi.P3D_M2M_vel(p, p["coord"], k, SIGMA)))
+        check_same(f"P3D dvort {k}", *both_legs(lambda: capi.P3D_M2M_dvort(p, t, k, SIGMA)))
+        if k in ("winckelmans", "gaussian"):
+            check_same(f"P3D visc {k}",
+                       *both_legs(lambda: capi.P3D_M2M_visc_dvort(p, t, k, SIGMA, 0.1)))
+        check_same(f"P2D vel {k}", *both_legs(lambda: capi.P2D_M2M_vel(q, q["coord"], k, SIGMA)))
+        if k in ("winckelmans", "gaussian"):
+            check_same(f"P2D visc {k}",
+                       *both_legs(lambda: capi.P2D_M2M_visc_dvort(q, tq, k, SIGMA, 0.1)))
+    for name, fn, ref in (
+            ("F3D vel", lambda: capi.F3D_M2M_vel(f, p["coord"]),
+             lambda: orc.F3D_M2M_vel(f, p["coord"])),
+            ("F3D dvort", lambda: capi.F3D_M2M_dvort(f, t), lambda: orc.F3D_M2M_dvort(f, t))):
         gpu, cpu = both_legs(fn)
         ref_worst, _ = worst_rel(ref(), cpu)
         print(f"  {name}: the reference's FP32 arithmetic, max |a-b|/|a+b| {ref_worst:.2e}")
