@@ -318,6 +318,12 @@ def bench_cube(args, cv, D, W, o, stream):
     line["steps"] = steps
     rf = roofline(pairs, F, ms_k)
     rf["hbm_gbps_algorithmic"] = algo_bytes / (ms_k * 1e-3) / 1e9
+    # HBM bytes of one evaluation (pair kernel + partition reduction),
+    # counter-measured; the small problem's rate against the 8 TB/s peak
+    rf["traffic"] = traffic_for("p3d_cube_vel_winckelmans", n)
+    if rf["traffic"]:
+        rf["hbm_gbps_measured"] = rf["traffic"] / (ms_step * 1e-3) / 1e9
+        rf["hbm_frac"] = rf["hbm_gbps_measured"] / 8000.0
     line.update({"roofline": rf, "end_to_end_c_api_ms": e2e_ms,
                  "rel_err_vs_fp64": {"samples": n, "vel": err_stats(V, v64),
                                      "c_api_vel": err_stats(Ve2e, v64)},
@@ -514,7 +520,9 @@ def bench_clouds(args, cv, D, W, o, stream):
                       "shape": {"targets_per_lane": tuned[0], "workgroup_size": tuned[1],
                                 "workgroups_per_cu": tuned[2], "tuned_by": "cvtx_hip_batch_autotune",
                                 "tuning_s": tune_s, "shape_before_tuning": list(shape0)}})
-    line.update({"roofline": roofline(pairs, F_CLOUDS, ms_k),
+    rf = roofline(pairs, F_CLOUDS, ms_k)
+    rf["traffic"] = traffic_for("p3d_clouds_vel_dvort_winckelmans", n)
+    line.update({"roofline": rf,
                  "per_cloud_plain_launches": {
                      "what": "one launch per cloud, 1 target per lane, no source partitions, "
                              "stream synchronised after each cloud",
