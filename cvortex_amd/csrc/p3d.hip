@@ -69,6 +69,22 @@ __global__ __launch_bounds__(256) void p3d_reduce(P3DKParams p) {
   p3d_finish<MASK>(d, at, q.w, p.sc, p.out_vel, p.out_dvort, p.out_visc, t);
 }
 
+// Velocity alone: one thread per target and component, so three times the
+// threads of p3d_reduce (a small problem's reduction has few targets), each
+// with sixteen partitions' loads in flight; the partitions are added in
+// order, as p3d_reduce does.  Cube of 8192 (32 partitions): 44.0 -> 43.6 us
+// per evaluation.
+__global__ __launch_bounds__(64) void p3d_reduce_vel(P3DKParams p) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= p.n_tgt) return;
+  double d = 0.0;
+#pragma unroll 16
+  for (int s = 0; s < p.n_split; ++s) d += p.partial[((size_t)s * 3 + c) * p.n_tgt + i];
+  const int t = p.tgt_perm ? p.tgt_perm[i] : i;
+  p.out_vel[3 * (size_t)t + c] = (float)(p.sc.vel * d);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -185,7 +201,6 @@ KernelFn pick_kernel(int reg, int mask, int acc, int T) {
 
 KernelFn pick_reduce(int mask) {
   switch (mask) {
-    case OUT_VEL: return p3d_reduce<OUT_VEL>;
     case OUT_DVORT: return p3d_reduce<OUT_DVORT>;
     case OUT_VISC: return p3d_reduce<OUT_VISC>;
     case OUT_VEL | OUT_DVORT: return p3d_reduce<OUT_VEL | OUT_DVORT>;
@@ -244,8 +259,12 @@ hipError_t launch_p3d(const P3DJob& j, const LaunchCfg& c, hipStream_t s) {
   hipLaunchKernelGGL(k, grid, dim3(c.wg), 0, s, p);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || n_split == 1) return e;
-  KernelFn r = pick_reduce(j.mask);
-  hipLaunchKernelGGL(r, dim3((j.n_tgt + 255) / 256), dim3(256), 0, s, p);
+  if (j.mask == OUT_VEL) {
+    hipLaunchKernelGGL(p3d_reduce_vel, dim3((j.n_tgt + 63) / 64, 3), dim3(64), 0, s, p);
+  } else {
+    KernelFn r = pick_reduce(j.mask);
+    hipLaunchKernelGGL(r, dim3((j.n_tgt + 255) / 256), dim3(256), 0, s, p);
+  }
   return hipGetLastError();
 }
 
