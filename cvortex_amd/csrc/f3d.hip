@@ -254,6 +254,7 @@ __global__ __launch_bounds__(WG, (f3d_min_waves<MASK, T>())) void f3d_kernel(F3D
       if constexpr ((MASK & OUT_DVORT) == 0) return in_vgpr(__builtin_inff());
       else return __builtin_inff();
     }();
+    const float c126 = in_vgpr(0x1p126f);
     float l2max;
     const Box<3> fbox = box_combine<3, WAVES>(s_box, &l2max);
     // every pair same-side and finite, and each of the N values that share
@@ -377,7 +378,12 @@ __global__ __launch_bounds__(WG, (f3d_min_waves<MASK, T>())) void f3d_kernel(F3D
             }
             if constexpr ((MASK & OUT_DVORT) != 0) {
               const float B = -fb[u].w * cn[q] * p12[q] * y12[q] * inv[FF::NP + q];
-              const float kd = cn[q] > 0.f ? kk : 0.f;
+              // kk, or 0 when c = 0: |c| is 0 or at least 2^-75 (the root
+              // of a float), so |c| 2^126 clamped to [0, 1] is 0 or 1 (one
+              // multiplication with the clamp modifier, where a compare does
+              // not dual-issue): lattice fused 19.18 -> 19.03 ms
+              // (kk is finite in far tiles, so kk * 0 is 0)
+              const float kd = kk * __builtin_amdgcn_fmed3f(cn[q] * c126, 0.f, 1.f);
               acc[k][D + 0] += B;
               acc[k][D + 1] = __builtin_fmaf(kd, f0[u].x, acc[k][D + 1]);
               acc[k][D + 2] = __builtin_fmaf(kd, f0[u].y, acc[k][D + 2]);
