@@ -11,7 +11,8 @@ The checks do not depend on the problem size:
       - a negative regularisation radius negates dvort exactly (the
         reference's signed sigma^3, src/P3D.cpp:102);
       - the persistent multi-cloud launch equals one launch per cloud;
-      - planetary PSE is exactly zero (reference src/VortFunc.cpp:52-62).
+      - planetary PSE is exactly zero (reference src/VortFunc.cpp:52-62), and
+        so is the Gaussian exchange of the sheet's equal strengths and areas.
 Workload generators: cvortex_amd/workloads.py."""
 import numpy as np
 import pytest
@@ -150,3 +151,10 @@ def test_sheet_2pow22(orc):
     err = oracle.rel_l2(v[idx], v64)
     print(f"  vel: relL2 vs FP64 {err:.2e}", flush=True)
     assert np.all(np.isfinite(v)) and err <= TOL
+    case("sheet: Gaussian vel + visc; equal strengths and areas make every exchange term 0")
+    vg, eg = cv.P2D_M2M_fused(p, p, "gaussian", sigma, nu)
+    assert np.all(eg == 0.0)
+    g64, _ = orc.f64_P2D(p, p, "gaussian", sigma, idx=idx)
+    err = oracle.rel_l2(vg[idx], g64)
+    print(f"  gaussian vel: relL2 vs FP64 {err:.2e}", flush=True)
+    assert np.all(np.isfinite(vg)) and err <= TOL
