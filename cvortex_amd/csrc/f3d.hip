@@ -271,12 +271,15 @@ __global__ __launch_bounds__(WG, (f3d_min_waves<MASK, T>())) void f3d_kernel(F3D
     if (far) {
       constexpr int U = FF::U;
       int j = 0;
-      // the last end point of a chained group, kept for the next group when
-      // the chain runs on into it (its first filament starts there, bit for
-      // bit, so the values are those the next group would compute).  With the
+      // the last end point of a group, kept for the next group when the
+      // chain runs on into it (its first filament starts there, bit for bit,
+      // so the values are those the next group would compute).  With the
       // stretching (two filaments per group) this saves a quarter of the end
       // points' rsq: lattice dvort 18.67 -> 17.94 ms, fused 19.74 -> 19.22;
-      // velocity alone (four per group) gained nothing and does without.
+      // both paths leave the point, which keeps the register allocator from
+      // copying it around the branch (dvort 18.07 -> 17.23, fused 19.14 ->
+      // 18.62).  Velocity alone (four per group) gained nothing and does
+      // without.
       constexpr bool CARRY = (MASK & OUT_DVORT) != 0;
       float lpx[T], lpy[T], lpz[T], lpns[T], lpy1[T];
       bool carry = false;
@@ -351,7 +354,6 @@ __global__ __launch_bounds__(WG, (f3d_min_waves<MASK, T>())) void f3d_kernel(F3D
           if constexpr (CARRY)
             carry = __builtin_amdgcn_readfirstlane(__float_as_int(f0[U - 1].w)) != 0;
         } else {
-          carry = false;
 #pragma unroll
           for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -360,9 +362,16 @@ __global__ __launch_bounds__(WG, (f3d_min_waves<MASK, T>())) void f3d_kernel(F3D
               const float r2x = xt[k].x - fb[u].x, r2y = xt[k].y - fb[u].y, r2z = xt[k].z - fb[u].z;
               const float n1s = __builtin_fmaf(r1x, r1x, __builtin_fmaf(r1y, r1y, r1z * r1z));
               const float n2s = __builtin_fmaf(r2x, r2x, __builtin_fmaf(r2y, r2y, r2z * r2z));
+              const float y2 = __builtin_amdgcn_rsqf(n2s);
               terms(u * T + k, u, r1x, r1y, r1z, r2x, r2y, r2z, n1s, n2s,
-                    __builtin_amdgcn_rsqf(n1s), __builtin_amdgcn_rsqf(n2s));
+                    __builtin_amdgcn_rsqf(n1s), y2);
+              if (u == U - 1) {
+                // the group's last end point, as the chained walk leaves it
+                lpx[k] = r2x; lpy[k] = r2y; lpz[k] = r2z; lpns[k] = n2s; lpy1[k] = y2;
+              }
             }
+          if constexpr (CARRY)
+            carry = __builtin_amdgcn_readfirstlane(__float_as_int(f0[U - 1].w)) != 0;
         }
         batch_rcp<FF::N>(den, inv);
 #pragma unroll
