@@ -278,9 +278,8 @@ __global__ __launch_bounds__(WG, (f3d_min_waves<MASK, T>())) void f3d_kernel(F3D
       // points' rsq: lattice dvort 18.67 -> 17.94 ms, fused 19.74 -> 19.22;
       // both paths leave the point, which keeps the register allocator from
       // copying it around the branch (dvort 18.07 -> 17.23, fused 19.14 ->
-      // 18.62).  Velocity alone (four per group) gained nothing and does
-      // without.
-      constexpr bool CARRY = (MASK & OUT_DVORT) != 0;
+      // 18.62), and then the velocity alone (four filaments per group, a
+      // fifth of its rsq saved) gains too: 10.87 -> 10.58 ms.
       float lpx[T], lpy[T], lpz[T], lpns[T], lpy1[T];
       bool carry = false;
       for (; j + U <= cnt; j += U, jv += U) {
@@ -326,7 +325,7 @@ __global__ __launch_bounds__(WG, (f3d_min_waves<MASK, T>())) void f3d_kernel(F3D
           chained = __builtin_amdgcn_readfirstlane(f) != 0;
         }
         if (chained) {
-          if (!(CARRY && carry)) {
+          if (!carry) {
 #pragma unroll
             for (int k = 0; k < T; ++k) {
               lpx[k] = xt[k].x - fa[0].x;
@@ -351,8 +350,7 @@ __global__ __launch_bounds__(WG, (f3d_min_waves<MASK, T>())) void f3d_kernel(F3D
             }
             lpx[k] = px; lpy[k] = py; lpz[k] = pz; lpns[k] = pns; lpy1[k] = py1;
           }
-          if constexpr (CARRY)
-            carry = __builtin_amdgcn_readfirstlane(__float_as_int(f0[U - 1].w)) != 0;
+          carry = __builtin_amdgcn_readfirstlane(__float_as_int(f0[U - 1].w)) != 0;
         } else {
 #pragma unroll
           for (int u = 0; u < U; ++u)
@@ -370,8 +368,7 @@ __global__ __launch_bounds__(WG, (f3d_min_waves<MASK, T>())) void f3d_kernel(F3D
                 lpx[k] = r2x; lpy[k] = r2y; lpz[k] = r2z; lpns[k] = n2s; lpy1[k] = y2;
               }
             }
-          if constexpr (CARRY)
-            carry = __builtin_amdgcn_readfirstlane(__float_as_int(f0[U - 1].w)) != 0;
+          carry = __builtin_amdgcn_readfirstlane(__float_as_int(f0[U - 1].w)) != 0;
         }
         batch_rcp<FF::N>(den, inv);
 #pragma unroll
