@@ -36,7 +36,8 @@ struct P3DLayout {
 // SPLIT (Gaussian with ACC_COMP): only pairs with rho^2 < 208 go to FP64,
 // inside the regularisation's near branch; the stretching terms of the
 // pairs beyond (Kv < 208^-3/2 in reduced units, and eta = 0 there) are
-// summed in FP32 tiles combined by TwoSum, in the F_A slots.
+// summed in FP32 tiles combined by TwoSum, in the F_A slots, on the far
+// branch (RegKernel<REG_GAUSSIAN>::eval_split).
 template <int REG, int MASK, int ACC>
 struct P3DRegs {
   static constexpr bool DBL = (ACC == ACC_COMP);
@@ -78,7 +79,27 @@ __device__ __forceinline__ void p3d_pair(const float4 sp, const float4 sa, const
   const float dz = xt.z - sp.z;
   const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
   float Kv, Kh, eta, Kfar = 0.f;
-  if constexpr (R::SPLIT) {
+  if constexpr (R::SPLIT && (MASK & OUT_DVORT) != 0) {
+    RegKernel<REG_GAUSSIAN>::eval_split(r2, rc, tab, Kv, Kh, eta,
+                                        [&](float kn, float en) {
+      const double2 sxy = *pxy, szv = *pzv;
+      const double k = (double)kn;
+      dacc[R::D_A + 0] = __builtin_fma(k, sxy.x, dacc[R::D_A + 0]);
+      dacc[R::D_A + 1] = __builtin_fma(k, sxy.y, dacc[R::D_A + 1]);
+      dacc[R::D_A + 2] = __builtin_fma(k, szv.x, dacc[R::D_A + 2]);
+      if constexpr ((MASK & OUT_VISC) != 0) {
+        const double e = (double)en;
+        dacc[R::D_E + 0] = __builtin_fma(e, sxy.x, dacc[R::D_E + 0]);
+        dacc[R::D_E + 1] = __builtin_fma(e, sxy.y, dacc[R::D_E + 1]);
+        dacc[R::D_E + 2] = __builtin_fma(e, szv.x, dacc[R::D_E + 2]);
+        dacc[R::D_E + 3] = __builtin_fma(e, szv.y, dacc[R::D_E + 3]);
+      }
+    }, [&](float kf) {
+      acc[R::F_A + 0] = __builtin_fmaf(kf, sa.x, acc[R::F_A + 0]);
+      acc[R::F_A + 1] = __builtin_fmaf(kf, sa.y, acc[R::F_A + 1]);
+      acc[R::F_A + 2] = __builtin_fmaf(kf, sa.z, acc[R::F_A + 2]);
+    });
+  } else if constexpr (R::SPLIT) {
     constexpr bool PIN = (MASK & (OUT_DVORT | OUT_VISC)) == 0;  // no FP64 near sums
     RegKernel<REG_GAUSSIAN>::template eval_ranges<PIN>(r2, rc, tab, Kv, Kh, eta, Kfar,
                                                        [&](float kn, float en) {
@@ -114,9 +135,7 @@ __device__ __forceinline__ void p3d_pair(const float4 sp, const float4 sa, const
     }
     if constexpr ((MASK & OUT_DVORT) != 0) {
       if constexpr (R::SPLIT) {
-        acc[R::F_A + 0] = __builtin_fmaf(Kfar, sa.x, acc[R::F_A + 0]);
-        acc[R::F_A + 1] = __builtin_fmaf(Kfar, sa.y, acc[R::F_A + 1]);
-        acc[R::F_A + 2] = __builtin_fmaf(Kfar, sa.z, acc[R::F_A + 2]);
+        // the far range's stretching sum was added by eval_split's callback
       } else if constexpr (R::DBL) {
         const double2 sxy = *pxy, szv = *pzv;
         const double k = (double)Kv;
