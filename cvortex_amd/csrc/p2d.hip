@@ -229,7 +229,7 @@ __global__ __launch_bounds__(WG) void p2d_kernel(P2DKParams p) {
         constexpr int U = 4 / T;
         const float ks = in_vgpr(P2DFar<REG>::kscale(p.rc));
         // a_s times the factor of 1/r^2 in K (planetary: 1, so no multiply)
-        auto sk = [ks](float a) {
+        auto sk = [=](float a) {
           if constexpr (REG == REG_PLANETARY) return a;
           else return a * ks;
         };
