@@ -218,9 +218,9 @@ __global__ __launch_bounds__(WG, (f3d_min_waves<MASK, T>())) void f3d_kernel(F3D
         int link = 0;
         if (tid + 1 < cnt) {
           const float4 an = p.fa[ts + tid + 1];
-          link = (__float_as_uint(an.x) == __float_as_uint(b.x)) &
-                 (__float_as_uint(an.y) == __float_as_uint(b.y)) &
-                 (__float_as_uint(an.z) == __float_as_uint(b.z));
+          link = (int)(__float_as_uint(an.x) == __float_as_uint(b.x)) &
+                 (int)(__float_as_uint(an.y) == __float_as_uint(b.y)) &
+                 (int)(__float_as_uint(an.z) == __float_as_uint(b.z));
         }
         s_a[tid] = make_float4(a.x, a.y, a.z, g);
         s_b[tid] = make_float4(b.x, b.y, b.z, 3.f * g / len);
