@@ -143,11 +143,15 @@ struct F3DFar {
 // Waves per SIMD the register allocation must leave room for.  The fused
 // kernel with one target per lane is held to 5 (at most 96 VGPRs; it takes
 // 103 and 4 waves unbounded, and spills 16 bytes held to 5): 16384
-// filaments x 2^20 targets, 21.29 -> 19.69 ms.  The single-output kernels
-// run best unbounded (vel 10.65 ms at 5 waves against 10.85 held to 7).
+// filaments x 2^20 targets, 21.29 -> 19.69 ms.  The velocity alone is held
+// to 6 (80 VGPRs instead of 89; 20 bytes spilled outside the source loops):
+// 10.37 -> 10.20 ms (three A/B pairs on one box); held to 7 it was slower.
+// The stretching alone runs best unbounded (5 waves; 17.24 ms against 17.73
+// held to 6).
 template <int MASK, int T>
 constexpr int f3d_min_waves() {
-  return (MASK == (OUT_VEL | OUT_DVORT) && T == 1) ? 5 : 1;
+  if constexpr (T != 1) return 1;
+  return MASK == (OUT_VEL | OUT_DVORT) ? 5 : (MASK == OUT_VEL ? 6 : 1);
 }
 
 template <int MASK, int ACC, int T, int WG>
