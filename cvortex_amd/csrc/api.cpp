@@ -168,29 +168,38 @@ int p3d_host(const cvtx_P3D** src, int ns, const bsv_V3f* mes, const cvtx_P3D** 
   if (!d) return -1;
   StreamUse use(*d, d->stream);
 
-  const size_t sb = sizeof(float4) * (size_t)(ns > 0 ? ns : 1);
-  const size_t tb = sizeof(float4) * (size_t)nt;
   const size_t ob = sizeof(float) * 3 * (size_t)nt;
   const bool need_ta = (mask & (OUT_DVORT | OUT_VISC)) != 0;
-  float4* h_spv = (float4*)host_buffer(rt, *d, SLOT_SRC0, sb);
-  float4* h_sa = (float4*)host_buffer(rt, *d, SLOT_SRC1, sb);
-  float4* h_tpv = (float4*)host_buffer(rt, *d, SLOT_TGT0, tb);
-  float4* h_ta = need_ta ? (float4*)host_buffer(rt, *d, SLOT_TGT1, tb) : nullptr;
-  float4* d_spv = (float4*)device_buffer(rt, *d, SLOT_SRC0, sb);
-  float4* d_sa = (float4*)device_buffer(rt, *d, SLOT_SRC1, sb);
-  float4* d_tpv = (float4*)device_buffer(rt, *d, SLOT_TGT0, tb);
-  float4* d_ta = need_ta ? (float4*)device_buffer(rt, *d, SLOT_TGT1, tb) : nullptr;
+  // sources {pv, a} and targets {pv, a} in one staging buffer, so that one
+  // copy moves them (each copy costs microseconds of latency, which is most
+  // of a small problem's call)
+  const size_t ns1 = (size_t)(ns > 0 ? ns : 1);
+  const size_t n4 = 2 * ns1 + (size_t)nt * (need_ta ? 2 : 1);
+  float4* h_in = (float4*)host_buffer(rt, *d, SLOT_SRC0, sizeof(float4) * n4);
+  float4* d_in = (float4*)device_buffer(rt, *d, SLOT_SRC0, sizeof(float4) * n4);
+  if (!h_in || !d_in) return -1;
+  float4* h_spv = h_in;
+  float4* h_sa = h_in + ns1;
+  float4* h_tpv = h_in + 2 * ns1;
+  float4* h_ta = need_ta ? h_tpv + nt : nullptr;
+  float4* d_spv = d_in;
+  float4* d_sa = d_in + ns1;
+  float4* d_tpv = d_in + 2 * ns1;
+  float4* d_ta = need_ta ? d_tpv + nt : nullptr;
+  // the selected outputs side by side, so that one copy brings them back
+  const int n_out = ((mask & OUT_VEL) ? 1 : 0) + ((mask & OUT_DVORT) ? 1 : 0) + ((mask & OUT_VISC) ? 1 : 0);
+  float* h_outs = (float*)host_buffer(rt, *d, SLOT_OUT0, ob * n_out);
+  float* d_outs = (float*)device_buffer(rt, *d, SLOT_OUT0, ob * n_out);
+  if (!h_outs || !d_outs) return -1;
   float* h_out[3] = {nullptr, nullptr, nullptr};
   float* d_out[3] = {nullptr, nullptr, nullptr};
   float* user_out[3] = {vel, dvort, visc};
-  for (int o = 0; o < 3; ++o) {
+  for (int o = 0, k = 0; o < 3; ++o) {
     if (!(mask & (1 << o))) continue;
-    h_out[o] = (float*)host_buffer(rt, *d, SLOT_OUT0 + o, ob);
-    d_out[o] = (float*)device_buffer(rt, *d, SLOT_OUT0 + o, ob);
-    if (!h_out[o] || !d_out[o]) return -1;
+    h_out[o] = h_outs + (size_t)k * 3 * nt;
+    d_out[o] = d_outs + (size_t)k * 3 * nt;
+    ++k;
   }
-  if (!h_spv || !h_sa || !h_tpv || !d_spv || !d_sa || !d_tpv || (need_ta && (!h_ta || !d_ta)))
-    return -1;
 
   gather_p3d(src, ns, h_spv, h_sa);
   if (mes) {
@@ -198,10 +207,7 @@ int p3d_host(const cvtx_P3D** src, int ns, const bsv_V3f* mes, const cvtx_P3D** 
   } else {
     gather_p3d(ind, nt, h_tpv, h_ta);
   }
-  if (!h2d(rt, *d, d_spv, h_spv, sizeof(float4) * ns)) return -1;
-  if (!h2d(rt, *d, d_sa, h_sa, sizeof(float4) * ns)) return -1;
-  if (!h2d(rt, *d, d_tpv, h_tpv, tb)) return -1;
-  if (need_ta && !h2d(rt, *d, d_ta, h_ta, tb)) return -1;
+  if (!h2d(rt, *d, d_in, h_in, sizeof(float4) * n4)) return -1;
 
   P3DJob j;
   j.src_pv = d_spv;
@@ -227,8 +233,7 @@ int p3d_host(const cvtx_P3D** src, int ns, const bsv_V3f* mes, const cvtx_P3D** 
   }
   if (!maybe_order(rt, *d, j, d->stream)) return -1;
   if (!check_hip(rt, launch_p3d(j, c, d->stream), "P3D kernel launch")) return -1;
-  for (int o = 0; o < 3; ++o)
-    if (d_out[o] && !d2h(rt, *d, h_out[o], d_out[o], ob)) return -1;
+  if (!d2h(rt, *d, h_outs, d_outs, ob * n_out)) return -1;
   if (!check_hip(rt, hipStreamSynchronize(d->stream), "P3D kernels")) return -1;
   for (int o = 0; o < 3; ++o)
     if (d_out[o]) std::memcpy(user_out[o], h_out[o], ob);
