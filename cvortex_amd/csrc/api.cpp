@@ -252,39 +252,42 @@ int f3d_host(const cvtx_F3D** src, int nf, const bsv_V3f* mes, const cvtx_P3D** 
   Device* d = active_device(rt);
   if (!d) return -1;
   StreamUse use(*d, d->stream);
-  const size_t sb = sizeof(float4) * (size_t)(nf > 0 ? nf : 1);
-  const size_t tb = sizeof(float4) * (size_t)nt;
   const size_t ob = sizeof(float) * 3 * (size_t)nt;
   const bool need_ta = (mask & OUT_DVORT) != 0;
-  float4* h_fa = (float4*)host_buffer(rt, *d, SLOT_SRC0, sb);
-  float4* h_fb = (float4*)host_buffer(rt, *d, SLOT_SRC1, sb);
-  float4* h_tpv = (float4*)host_buffer(rt, *d, SLOT_TGT0, tb);
-  float4* h_ta = need_ta ? (float4*)host_buffer(rt, *d, SLOT_TGT1, tb) : nullptr;
-  float4* d_fa = (float4*)device_buffer(rt, *d, SLOT_SRC0, sb);
-  float4* d_fb = (float4*)device_buffer(rt, *d, SLOT_SRC1, sb);
-  float4* d_tpv = (float4*)device_buffer(rt, *d, SLOT_TGT0, tb);
-  float4* d_ta = need_ta ? (float4*)device_buffer(rt, *d, SLOT_TGT1, tb) : nullptr;
+  // filaments and targets in one staging buffer, the outputs side by side:
+  // one copy each way (as the P3D path)
+  const size_t nf1 = (size_t)(nf > 0 ? nf : 1);
+  const size_t n4 = 2 * nf1 + (size_t)nt * (need_ta ? 2 : 1);
+  const int n_out = ((mask & OUT_VEL) ? 1 : 0) + ((mask & OUT_DVORT) ? 1 : 0);
+  float4* h_in = (float4*)host_buffer(rt, *d, SLOT_SRC0, sizeof(float4) * n4);
+  float4* d_in = (float4*)device_buffer(rt, *d, SLOT_SRC0, sizeof(float4) * n4);
+  float* h_outs = (float*)host_buffer(rt, *d, SLOT_OUT0, ob * n_out);
+  float* d_outs = (float*)device_buffer(rt, *d, SLOT_OUT0, ob * n_out);
+  if (!h_in || !d_in || !h_outs || !d_outs) return -1;
+  float4* h_fa = h_in;
+  float4* h_fb = h_in + nf1;
+  float4* h_tpv = h_in + 2 * nf1;
+  float4* h_ta = need_ta ? h_tpv + nt : nullptr;
+  float4* d_fa = d_in;
+  float4* d_fb = d_in + nf1;
+  float4* d_tpv = d_in + 2 * nf1;
+  float4* d_ta = need_ta ? d_tpv + nt : nullptr;
   float* h_out[2] = {nullptr, nullptr};
   float* d_out[2] = {nullptr, nullptr};
   float* user_out[2] = {vel, dvort};
-  for (int o = 0; o < 2; ++o) {
+  for (int o = 0, k = 0; o < 2; ++o) {
     if (!(mask & (1 << o))) continue;
-    h_out[o] = (float*)host_buffer(rt, *d, SLOT_OUT0 + o, ob);
-    d_out[o] = (float*)device_buffer(rt, *d, SLOT_OUT0 + o, ob);
-    if (!h_out[o] || !d_out[o]) return -1;
+    h_out[o] = h_outs + (size_t)k * 3 * nt;
+    d_out[o] = d_outs + (size_t)k * 3 * nt;
+    ++k;
   }
-  if (!h_fa || !h_fb || !h_tpv || !d_fa || !d_fb || !d_tpv || (need_ta && (!h_ta || !d_ta)))
-    return -1;
   gather_f3d(src, nf, h_fa, h_fb);
   if (mes) {
     gather_pts3(mes, nt, h_tpv);
   } else {
     gather_p3d(ind, nt, h_tpv, h_ta);
   }
-  if (!h2d(rt, *d, d_fa, h_fa, sizeof(float4) * nf)) return -1;
-  if (!h2d(rt, *d, d_fb, h_fb, sizeof(float4) * nf)) return -1;
-  if (!h2d(rt, *d, d_tpv, h_tpv, tb)) return -1;
-  if (need_ta && !h2d(rt, *d, d_ta, h_ta, tb)) return -1;
+  if (!h2d(rt, *d, d_in, h_in, sizeof(float4) * n4)) return -1;
   F3DJob j;
   j.fa = d_fa;
   j.fb = d_fb;
@@ -307,8 +310,7 @@ int f3d_host(const cvtx_F3D** src, int nf, const bsv_V3f* mes, const cvtx_P3D** 
     if (!j.tgt_perm) return -1;
   }
   if (!check_hip(rt, launch_f3d(j, c, d->stream), "F3D kernel launch")) return -1;
-  for (int o = 0; o < 2; ++o)
-    if (d_out[o] && !d2h(rt, *d, h_out[o], d_out[o], ob)) return -1;
+  if (!d2h(rt, *d, h_outs, d_outs, ob * n_out)) return -1;
   if (!check_hip(rt, hipStreamSynchronize(d->stream), "F3D kernels")) return -1;
   for (int o = 0; o < 2; ++o)
     if (d_out[o]) std::memcpy(user_out[o], h_out[o], ob);
@@ -341,28 +343,32 @@ int p2d_host(const cvtx_P2D** src, int ns, const bsv_V2f* mes, const cvtx_P2D** 
   Device* d = active_device(rt);
   if (!d) return -1;
   StreamUse use(*d, d->stream);
-  const size_t sb = sizeof(float4) * (size_t)(ns > 0 ? ns : 1);
-  const size_t tb = sizeof(float4) * (size_t)nt;
-  float4* h_s = (float4*)host_buffer(rt, *d, SLOT_SRC0, sb);
-  float4* h_t = (float4*)host_buffer(rt, *d, SLOT_TGT0, tb);
-  float4* d_s = (float4*)device_buffer(rt, *d, SLOT_SRC0, sb);
-  float4* d_t = (float4*)device_buffer(rt, *d, SLOT_TGT0, tb);
-  const size_t vb = sizeof(float) * 2 * (size_t)nt, eb = sizeof(float) * (size_t)nt;
-  float* h_v = vel && (mask & OUT_VEL) ? (float*)host_buffer(rt, *d, SLOT_OUT0, vb) : nullptr;
-  float* d_v = vel && (mask & OUT_VEL) ? (float*)device_buffer(rt, *d, SLOT_OUT0, vb) : nullptr;
-  float* h_e = (mask & OUT_VISC) ? (float*)host_buffer(rt, *d, SLOT_OUT1, eb) : nullptr;
-  float* d_e = (mask & OUT_VISC) ? (float*)device_buffer(rt, *d, SLOT_OUT1, eb) : nullptr;
-  if (!h_s || !h_t || !d_s || !d_t) return -1;
-  if ((mask & OUT_VEL) && (!h_v || !d_v)) return -1;
-  if ((mask & OUT_VISC) && (!h_e || !d_e)) return -1;
+  // sources and targets in one staging buffer, the outputs side by side:
+  // one copy each way (as the P3D path)
+  const size_t ns1 = (size_t)(ns > 0 ? ns : 1);
+  const size_t n4 = ns1 + (size_t)nt;
+  const size_t vb = (mask & OUT_VEL) ? sizeof(float) * 2 * (size_t)nt : 0;
+  const size_t eb = (mask & OUT_VISC) ? sizeof(float) * (size_t)nt : 0;
+  float4* h_in = (float4*)host_buffer(rt, *d, SLOT_SRC0, sizeof(float4) * n4);
+  float4* d_in = (float4*)device_buffer(rt, *d, SLOT_SRC0, sizeof(float4) * n4);
+  float* h_outs = (float*)host_buffer(rt, *d, SLOT_OUT0, vb + eb);
+  float* d_outs = (float*)device_buffer(rt, *d, SLOT_OUT0, vb + eb);
+  if (!h_in || !d_in || !h_outs || !d_outs) return -1;
+  float4* h_s = h_in;
+  float4* h_t = h_in + ns1;
+  float4* d_s = d_in;
+  float4* d_t = d_in + ns1;
+  float* h_v = vb ? h_outs : nullptr;
+  float* d_v = vb ? d_outs : nullptr;
+  float* h_e = eb ? h_outs + vb / sizeof(float) : nullptr;
+  float* d_e = eb ? d_outs + vb / sizeof(float) : nullptr;
   gather_p2d(src, ns, h_s);
   if (mes) {
     gather_pts2(mes, nt, h_t);
   } else {
     gather_p2d(ind, nt, h_t);
   }
-  if (!h2d(rt, *d, d_s, h_s, sizeof(float4) * ns)) return -1;
-  if (!h2d(rt, *d, d_t, h_t, tb)) return -1;
+  if (!h2d(rt, *d, d_in, h_in, sizeof(float4) * n4)) return -1;
   P2DJob j;
   j.src = d_s;
   j.n_src = ns;
@@ -382,8 +388,7 @@ int p2d_host(const cvtx_P2D** src, int ns, const bsv_V2f* mes, const cvtx_P2D** 
     if (!j.partial) return -1;
   }
   if (!check_hip(rt, launch_p2d(j, c, d->stream), "P2D kernel launch")) return -1;
-  if (d_v && !d2h(rt, *d, h_v, d_v, vb)) return -1;
-  if (d_e && !d2h(rt, *d, h_e, d_e, eb)) return -1;
+  if (!d2h(rt, *d, h_outs, d_outs, vb + eb)) return -1;
   if (!check_hip(rt, hipStreamSynchronize(d->stream), "P2D kernels")) return -1;
   if (d_v) std::memcpy(vel, h_v, vb);
   if (d_e) std::memcpy(visc, h_e, eb);
