@@ -35,6 +35,27 @@ def test_influence_matrix_matches_oracle(orc, nf, nm):
                           orc.f64_F3D_inf_mtrx(fl, pts, dirs), TOL)
 
 
+@pytest.mark.parametrize("broken", [None, 77, 128])
+def test_influence_matrix_polyline(orc, broken):
+    # consecutive segments that share end points bit for bit (the kernel
+    # then computes each shared point's distance terms once per row), with
+    # optionally one link broken by one ulp
+    case(f"inf_mtrx polyline broken={broken}")
+    n = 301
+    rng = np.random.default_rng(11)
+    pts_line = np.cumsum(rng.normal(scale=0.05, size=(n + 1, 3)), axis=0).astype(np.float32)
+    a = pts_line[:-1].copy()
+    b = pts_line[1:].copy()
+    if broken is not None:
+        a[broken, 0] = np.nextafter(a[broken, 0], np.float32(np.inf))
+    fl = make_f3d(a, b, rng.normal(size=n).astype(np.float32))
+    pts = W.random_p3d(97, 5, box=1.0, vort=1.0)["coord"]
+    dirs = rng.normal(size=(97, 3)).astype(np.float32)
+    got = cv.F3D_inf_mtrx(fl, pts, dirs)
+    assert_parity_limited("inf_mtrx polyline", got, orc.F3D_inf_mtrx(fl, pts, dirs),
+                          orc.f64_F3D_inf_mtrx(fl, pts, dirs), TOL)
+
+
 def test_influence_matrix_vortex_lattice(orc):
     # the use the matrix is made for: a 16 x 16 lattice of square vortex
     # rings on z = 0, collocation points at the panel centres, normals +z
